@@ -1,0 +1,173 @@
+/*
+ * convgemm.h -- MI355X (gfx950) convolution as implicit GEMM.
+ *
+ * Drop-in for convGemm's C interface (src/convGemm.h:20-64): the same five
+ * symbols with the same argument order, layouts and semantics, except that
+ * every tensor pointer is a DEVICE pointer (hipMalloc'd memory on the current
+ * HIP device).  Each drop-in call launches on the library stream (the null
+ * stream unless convgemm_set_stream() chose another) and returns after the
+ * stream has finished, matching the reference's blocking behaviour.
+ *
+ * Layouts (identical to the reference):
+ *   NHWC: X, dX [b][h][w][c]   W, dW [c][kh][kw][kn]   Y, dY [b][ho][wo][kn]
+ *   NCHW: X, dX [b][c][h][w]   W, dW [kn][c][kh][kw]   Y, dY [b][kn][ho][wo]
+ *   ho = (h + 2*vpadding - vdilation*(kh-1) - 1)/vstride + 1, likewise wo.
+ *
+ * Error reporting: the drop-in functions return void like the reference;
+ * a rejected call (bad shape, NULL operand, HIP failure) writes nothing and
+ * records a status readable with convgemm_get_last_error() and
+ * convgemm_last_error_message().
+ */
+#ifndef CONVGEMM_AMD_CONVGEMM_H
+#define CONVGEMM_AMD_CONVGEMM_H
+
+#include <stdbool.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Drop-in entry points (replace src/convGemm.h:20-64)                       */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces alloc_pack_buffs (src/convGemm.h:20, src/convGemm.c:30-40).
+ * Allocates the two device workspaces used by the conv calls (split-K
+ * partial sums and the repacked weights).  Returns 0 on success, 1 if an
+ * allocation failed.  Release them with convgemm_free_pack_buffs(). */
+int alloc_pack_buffs(float **Ac_pack, float **Bc_pack);
+
+/* Replaces sconvGemmNHWC (src/convGemm.h:22-33, src/convGemm.c:48-77).
+ * trans == 'N': out = epi(conv(x, in)): in = W, x = X, out = Y, with the
+ *   optional fused epilogue t += bias; t = (t-mean)*inv_std*gamma + beta;
+ *   t = max(t, 0).  A NULL epilogue pointer means the term is absent.
+ * otherwise:    out = dW = im2row(x)^T * in: in = dY, x = X (epilogue
+ *   arguments ignored). */
+void sconvGemmNHWC(char trans,
+                   int b, int h, int w, int c,
+                   int kn, int kh, int kw,
+                   int vpadding, int hpadding,
+                   int vstride, int hstride,
+                   int vdilation, int hdilation,
+                   const float *in,
+                   const float *x,
+                   float *out, const float *bias_vector,
+                   const float *bn_running_mean, const float *bn_inv_std,
+                   const float *bn_gamma, const float *bn_beta, bool relu,
+                   float *ac_pack, float *bc_pack);
+
+/* Replaces sconvGemmNHWC_back (src/convGemm.h:35-42, src/convGemm.c:82-97).
+ * dx += row2im(weights^T * dy).  NOTE: stride comes before padding here. */
+void sconvGemmNHWC_back(int b, int h, int w, int c,
+                        int kn, int kh, int kw,
+                        int vstride, int hstride,
+                        int vpadding, int hpadding,
+                        int vdilation, int hdilation,
+                        const float *weights,
+                        const float *dy, float *dx,
+                        float *ac_pack, float *bc_pack);
+
+/* Replaces sconvGemmNCHW (src/convGemm.h:44-55, src/convGemm.c:99-125). */
+void sconvGemmNCHW(char trans,
+                   int b, int c, int h, int w,
+                   int kn, int kh, int kw,
+                   int vpadding, int hpadding,
+                   int vstride, int hstride,
+                   int vdilation, int hdilation,
+                   const float *in,
+                   const float *x,
+                   float *out, const float *bias_vector,
+                   const float *bn_running_mean, const float *bn_inv_std,
+                   const float *bn_gamma, const float *bn_beta, bool relu,
+                   float *ac_pack, float *bc_pack);
+
+/* Replaces sconvGemmNCHW_back (src/convGemm.h:57-64, src/convGemm.c:130-146).
+ * dx += col2im(dy * weights) (accumulates, like the reference). */
+void sconvGemmNCHW_back(int b, int c, int h, int w,
+                        int kn, int kh, int kw,
+                        int vstride, int hstride,
+                        int vpadding, int hpadding,
+                        int vdilation, int hdilation,
+                        const float *weights,
+                        const float *dy, float *dx,
+                        float *ac_pack, float *bc_pack);
+
+/* ------------------------------------------------------------------------ */
+/* Extended API                                                               */
+/* ------------------------------------------------------------------------ */
+
+typedef enum {
+    CONVGEMM_SUCCESS = 0,
+    CONVGEMM_ERR_INVALID_VALUE = 1,  /* bad shape, stride, trans, NULL operand */
+    CONVGEMM_ERR_ALLOC = 2,          /* device allocation failed */
+    CONVGEMM_ERR_HIP = 3,            /* a HIP runtime call failed */
+    CONVGEMM_ERR_WORKSPACE = 4,      /* workspace too small for the call */
+    CONVGEMM_ERR_UNSUPPORTED = 5     /* valid but not supported (e.g. kernel > 64 taps per side) */
+} convgemm_status_t;
+
+typedef enum { CONVGEMM_FWD = 0, CONVGEMM_WGRAD = 1, CONVGEMM_DGRAD = 2 } convgemm_op_t;
+typedef enum { CONVGEMM_NHWC = 0, CONVGEMM_NCHW = 1 } convgemm_layout_t;
+
+/* Operand storage type of `in` and `x`.  Accumulation is always FP32 and
+ * `out` is always FP32.  CONVGEMM_BF16: operands are bfloat16 (raw uint16).
+ * CONVGEMM_FP8_E4M3: operands are OCP e4m3fn bytes holding value/scale with
+ * per-tensor scales given in convgemm_conv_ex's `scales`. */
+typedef enum { CONVGEMM_F32 = 0, CONVGEMM_BF16 = 1, CONVGEMM_FP8_E4M3 = 2 } convgemm_dtype_t;
+
+typedef struct {
+    int op;      /* convgemm_op_t */
+    int layout;  /* convgemm_layout_t */
+    int b, h, w, c, kn, kh, kw;
+    int vpad, hpad, vstride, hstride, vdil, hdil;
+} convgemm_desc_t;
+
+typedef struct {
+    const float *bias, *mean, *inv_std, *gamma, *beta;
+    int relu;
+} convgemm_epilogue_t;
+
+/* hipStream_t, kept opaque so C callers need no HIP header. */
+typedef void *convgemm_stream_t;
+
+/* General form of the six operations, asynchronous on `stream`:
+ *   FWD:   out(Y)  = epi(alpha*conv(x=X, in=W) + beta*Y)
+ *   WGRAD: out(dW) = alpha*im2row(x=X)^T in(dY) + beta*dW
+ *   DGRAD: out(dX) = alpha*row2im(in(W)^T x(dY)) + beta*dX
+ * (the drop-in calls are alpha=1, beta=0 for FWD/WGRAD and beta=1 for DGRAD).
+ * alpha == 0 follows quick_gemm.h: out = beta*out (0 if beta == 0) and the
+ * epilogue is not applied.  `epi` may be NULL.  `workspace`/`ws_bytes` may
+ * be NULL/0, in which case the library uses an internal device buffer.
+ * `scales` (FP8 only): {scale_in, scale_x}, the true values are
+ * stored*scale. */
+convgemm_status_t convgemm_conv_ex(const convgemm_desc_t *desc, float alpha, float beta,
+                                   const void *in, const void *x, void *out,
+                                   const convgemm_epilogue_t *epi, convgemm_dtype_t dtype,
+                                   const float *scales,
+                                   void *workspace, size_t ws_bytes, convgemm_stream_t stream);
+
+/* Device workspace bytes convgemm_conv_ex needs for this problem. */
+size_t convgemm_workspace_size(const convgemm_desc_t *desc, convgemm_dtype_t dtype);
+
+/* Output spatial size with the reference's formula (src/convGemm.c:60-61). */
+int convgemm_out_size(int in, int pad, int dil, int k, int stride);
+
+/* Frees buffers from alloc_pack_buffs (the reference has no free function;
+ * its buffers came from aligned_alloc). */
+void convgemm_free_pack_buffs(float *Ac_pack, float *Bc_pack);
+
+/* Stream used by the drop-in calls (default: the null stream). */
+convgemm_status_t convgemm_set_stream(convgemm_stream_t stream);
+
+convgemm_status_t convgemm_get_last_error(void);
+const char *convgemm_last_error_message(void);
+const char *convgemm_status_string(convgemm_status_t status);
+
+/* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
+int convgemm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CONVGEMM_AMD_CONVGEMM_H */
