@@ -1,0 +1,248 @@
+#!/usr/bin/env python3
+"""Benchmark: FP32 implicit-GEMM convolution on one MI355X.
+
+Workload (one "step"): the ResNet-50-shaped layer sweep at batch 32 --
+L1 1x1 256->64 @56^2, L2 3x3 64->64 @56^2, L3 3x3 s2 128->128 56->28,
+L4 1x1 1024->256 @14^2, L5 3x3 512->512 @7^2 -- forward in both NHWC and NCHW
+with the fused bias + batch-norm + ReLU epilogue: 10 convolutions, 57.6 GFLOP
+(2*P*K*N, padding zeros counted as in plots/convgemm.plt:56-57).
+Inputs are synthetic (X ~ N(0,1), He-scaled weights, epilogue vectors as in
+the workload description) and resident in HBM before timing starts.
+
+Prints one JSON line:  value = TFLOP/s over the timed steps, the roofline
+of the dominant kernel measured with HIP events on the launch stream, the
+FP64 error of every layer, and a CPU baseline (the oracle's FP32
+restatement of the reference algorithm, timed on the host cores).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "conv TFLOP/s (% of roofline), BF16/FP8 vs FP32 plain at stated FP64 error"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+# name: (b, h, w, c, kn, kh, kw, pad, stride)
+LAYERS = [
+    ("L1_1x1_256to64_56", (32, 56, 56, 256, 64, 1, 1, 0, 1)),
+    ("L2_3x3_64to64_56", (32, 56, 56, 64, 64, 3, 3, 1, 1)),
+    ("L3_3x3s2_128to128_56", (32, 56, 56, 128, 128, 3, 3, 1, 2)),
+    ("L4_1x1_1024to256_14", (32, 14, 14, 1024, 256, 1, 1, 0, 1)),
+    ("L5_3x3_512to512_7", (32, 7, 7, 512, 512, 3, 3, 1, 1)),
+]
+
+
+def out_hw(h, pad, k, stride):
+    return (h + 2 * pad - (k - 1) - 1) // stride + 1
+
+
+def layer_flops(spec):
+    b, h, w, c, kn, kh, kw, pad, st = spec
+    ho, wo = out_hw(h, pad, kh, st), out_hw(w, pad, kw, st)
+    return 2.0 * b * ho * wo * c * kh * kw * kn
+
+
+def layer_bytes(spec):
+    """Compulsory HBM bytes of a forward conv: X + W + Y + 5 epilogue vectors (FP32)."""
+    b, h, w, c, kn, kh, kw, pad, st = spec
+    ho, wo = out_hw(h, pad, kh, st), out_hw(w, pad, kw, st)
+    return 4.0 * (b * h * w * c + c * kh * kw * kn + b * ho * wo * kn + 5 * kn)
+
+
+def make_inputs(torch, layout, spec, seed):
+    b, h, w, c, kn, kh, kw, pad, st = spec
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    ho, wo = out_hw(h, pad, kh, st), out_hw(w, pad, kw, st)
+    xs = (b, h, w, c) if layout == 0 else (b, c, h, w)
+    ws = (c, kh, kw, kn) if layout == 0 else (kn, c, kh, kw)
+    ys = (b, ho, wo, kn) if layout == 0 else (b, kn, ho, wo)
+    d = dict(
+        X=torch.randn(xs, device="cuda", generator=g),
+        W=torch.randn(ws, device="cuda", generator=g) * (2.0 / (c * kh * kw)) ** 0.5,
+        Y=torch.empty(ys, device="cuda"),
+        bias=0.1 * torch.randn(kn, device="cuda", generator=g),
+        mean=0.1 * torch.randn(kn, device="cuda", generator=g),
+        inv_std=1.0 / torch.sqrt(0.5 + 1.5 * torch.rand(kn, device="cuda", generator=g) + 1e-5),
+        gamma=0.5 + torch.rand(kn, device="cuda", generator=g),
+        bnbeta=0.1 * torch.randn(kn, device="cuda", generator=g),
+    )
+    return d
+
+
+def fp64_error(layout, spec, d, nsamp=512):
+    """max|y - y64| / max|y64| at sampled outputs (FP64 oracle, oracle/sampled.py)."""
+    from oracle import Shape, sampled
+    b, h, w, c, kn, kh, kw, pad, st = spec
+    s = Shape(b, h, w, c, kn, kh, kw, pad, pad, st, st, 1, 1)
+    rng = np.random.default_rng(123)
+    idx = np.stack([rng.integers(0, n, nsamp) for n in (s.b, s.ho, s.wo, s.kn)], 1)
+    host = {k: v.cpu().numpy() for k, v in d.items()}
+    ref = sampled.fwd_sampled(layout, s, host["W"], host["X"], idx, bias=host["bias"], mean=host["mean"],
+                              inv_std=host["inv_std"], gamma=host["gamma"], bnbeta=host["bnbeta"], relu=True)
+    got = sampled.nhwc_index(layout, host["Y"], idx)
+    return float(np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-30))
+
+
+def cpu_baseline(budget_s=12.0):
+    """Time the oracle's FP32 restatement of the reference's implicit GEMM
+    (oracle/convgemm_port.c) on the host cores, on the same layer sweep at
+    batch 1 (1/32 of a step), repeated until ~budget_s seconds have passed."""
+    import oracle
+    from oracle import Shape
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    threads = max(1, min(threads, len(os.sched_getaffinity(0))))
+    oracle.port_set_threads(threads)
+    rng = np.random.default_rng(7)
+    probs = []
+    for layout in (0, 1):
+        for name, spec in LAYERS:
+            _, h, w, c, kn, kh, kw, pad, st = spec
+            s = Shape(1, h, w, c, kn, kh, kw, pad, pad, st, st, 1, 1)
+            X = rng.standard_normal(s.x_shape(layout)).astype(np.float32)
+            W = (rng.standard_normal(s.w_shape(layout)) * np.sqrt(2.0 / s.K)).astype(np.float32)
+            v = lambda: rng.standard_normal(kn).astype(np.float32)
+            epi = dict(bias=v(), mean=v(), inv_std=np.abs(v()) + 0.5, gamma=np.abs(v()) + 0.5, bnbeta=v(), relu=True)
+            out = np.empty(s.y_shape(layout), np.float32)
+            probs.append((layout, s, W, X, epi, out))
+    flops_per_rep = sum(p[1].flops() for p in probs)
+    for p in probs:  # warm-up
+        oracle.port_conv(p[0], "N", p[1], p[2], p[3], out=p[5], **p[4])
+    reps = 0
+    t0 = time.perf_counter()
+    while True:
+        for p in probs:
+            oracle.port_conv(p[0], "N", p[1], p[2], p[3], out=p[5], **p[4])
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or reps >= 200:
+            break
+    return {"value": flops_per_rep * reps / el / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
+            "sample": f"same 10-convolution sweep at batch 1 (1/32 of a step), {reps} reps in {el:.1f} s, "
+                      f"FP32 restatement of the reference's B3A2C0 implicit GEMM (portable micro-kernel, "
+                      f"OpenMP {threads} threads)"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-error", action="store_true", help="skip the FP64 error check")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py: this build targets one GPU; --gpus {a.gpus} is not supported "
+              "(the convolution is sharded by batch across GPUs only in design, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+    import convgemm_amd as cg
+
+    assert torch.cuda.is_available(), "bench.py needs an MI355X"
+    cg.lib()
+    stream = torch.cuda.current_stream()
+    convs = []
+    for layout in (0, 1):
+        for li, (name, spec) in enumerate(LAYERS):
+            b, h, w, c, kn, kh, kw, pad, st = spec
+            d = make_inputs(torch, layout, spec, 0xC0FFEE + 2000 + 10 * li + layout)
+            desc = cg.make_desc(cg.FWD, layout, b, h, w, c, kn, kh, kw, pad, pad, st, st)
+            convs.append(dict(name=f"{name}_{'NHWC' if layout == 0 else 'NCHW'}", layout=layout, spec=spec,
+                              desc=desc, d=d, flops=layer_flops(spec), bytes=layer_bytes(spec)))
+
+    def run(cv):
+        d = cv["d"]
+        cg.conv_ex(cv["desc"], d["W"], d["X"], d["Y"], bias=d["bias"], mean=d["mean"], inv_std=d["inv_std"],
+                   gamma=d["gamma"], bnbeta=d["bnbeta"], relu=True, stream=stream)
+
+    for cv in convs:
+        run(cv)
+        cv["plan"] = cg.last_plan()
+    for _ in range(a.warmup):
+        for cv in convs:
+            run(cv)
+    torch.cuda.synchronize()
+
+    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in convs]
+          for _ in range(a.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        for i, cv in enumerate(convs):
+            ev[k][i][0].record(stream)
+            run(cv)
+            ev[k][i][1].record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+
+    step_flops = sum(cv["flops"] for cv in convs)
+    elapsed = t1 - t0
+    value = step_flops * a.steps / elapsed / 1e12
+    per = []
+    for i, cv in enumerate(convs):
+        ms = float(np.mean([ev[k][i][0].elapsed_time(ev[k][i][1]) for k in range(a.steps)]))
+        per.append(ms)
+        cv["ms"] = ms
+    # dominant kernel: the largest share of the step among calls that launch one kernel
+    single = [cv for cv in convs if cv["plan"][2] == 1] or convs
+    dom = max(single, key=lambda cv: cv["ms"])
+    achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_fwd_fp32.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get(dom["name"])
+        except Exception:
+            traffic = None
+
+    errors = {}
+    if not a.no_error:
+        for cv in convs:
+            errors[cv["name"]] = fp64_error(cv["layout"], cv["spec"], cv["d"])
+
+    layers = {cv["name"]: {"ms": round(cv["ms"], 4), "tflops": round(cv["flops"] / (cv["ms"] * 1e-3) / 1e12, 2),
+                           "frac_fp32_peak": round(cv["flops"] / (cv["ms"] * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 3),
+                           "plan": {"cfg": cv["plan"][0], "splits": cv["plan"][1], "kernels": cv["plan"][2]},
+                           "fp64_max_rel_err": errors.get(cv["name"])}
+              for cv in convs}
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(elapsed / a.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "resnet50_layer_sweep_fwd_b32_nhwc_nchw_bias_bn_relu",
+                   "layers": [n for n, _ in LAYERS], "layouts": ["NHWC", "NCHW"], "global_batch": 32,
+                   "gflop_per_step": round(step_flops / 1e9, 3)},
+        "roofline": {"bound": "mfma", "kernel": dom["name"], "achieved": round(achieved, 3),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": traffic},
+        "fp64_max_rel_err": max(errors.values()) if errors else None,
+        "layers": layers,
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,239 @@
+"""convgemm_amd -- MI355X (gfx950) convolution as implicit GEMM.
+
+Python mirror of convGemm's C interface (src/convGemm.h:20-64) over the
+C-ABI library ``libconvgemm.so`` (include/convgemm.h).  The functions take
+torch tensors on the GPU (or raw device addresses as ints) and call the HIP
+kernels through ctypes; torch is only used for device memory and streams.
+
+The drop-in functions keep the reference's names, argument order and
+meaning.  Where the reference has no error reporting (it returns void and
+does no validation), these wrappers raise :class:`ConvGemmError` when the
+library records a failure.
+
+There is no CPU fallback: if ``libconvgemm.so`` is missing this module
+raises on first use.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+__all__ = [
+    "ConvGemmError", "lib", "alloc_pack_buffs", "free_pack_buffs", "sconvGemmNHWC", "sconvGemmNHWC_back",
+    "sconvGemmNCHW", "sconvGemmNCHW_back", "conv_ex", "workspace_size", "out_size", "force_config",
+    "FWD", "WGRAD", "DGRAD", "NHWC", "NCHW", "F32", "BF16", "FP8_E4M3", "EXPORTED_SYMBOLS",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libconvgemm.so")
+
+FWD, WGRAD, DGRAD = 0, 1, 2
+NHWC, NCHW = 0, 1
+F32, BF16, FP8_E4M3 = 0, 1, 2
+
+STATUS = {0: "success", 1: "invalid value", 2: "device allocation failed", 3: "HIP runtime error",
+          4: "workspace too small", 5: "unsupported problem"}
+
+# Every symbol include/convgemm.h declares.
+EXPORTED_SYMBOLS = (
+    "alloc_pack_buffs", "sconvGemmNHWC", "sconvGemmNHWC_back", "sconvGemmNCHW", "sconvGemmNCHW_back",
+    "convgemm_conv_ex", "convgemm_workspace_size", "convgemm_out_size", "convgemm_free_pack_buffs",
+    "convgemm_set_stream", "convgemm_get_last_error", "convgemm_last_error_message", "convgemm_status_string",
+    "convgemm_version",
+)
+
+
+class ConvGemmError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"convgemm: {STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("op", "layout", "b", "h", "w", "c", "kn", "kh", "kw", "vpad", "hpad",
+                                             "vstride", "hstride", "vdil", "hdil")]
+
+
+class Epilogue(ctypes.Structure):
+    _fields_ = [("bias", ctypes.c_void_p), ("mean", ctypes.c_void_p), ("inv_std", ctypes.c_void_p),
+                ("gamma", ctypes.c_void_p), ("beta", ctypes.c_void_p), ("relu", ctypes.c_int)]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libconvgemm.so (after torch, so both share one HIP runtime)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m convgemm_amd.build` "
+                          "(there is no CPU fallback)")
+    try:  # torch ships libamdhip64.so.7; loading it first makes ours bind to the same runtime
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ci, cf = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+    L.alloc_pack_buffs.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp)]
+    L.alloc_pack_buffs.restype = ci
+    L.sconvGemmNHWC.argtypes = [ctypes.c_char] + [ci] * 13 + [vp] * 8 + [ctypes.c_bool, vp, vp]
+    L.sconvGemmNHWC.restype = None
+    L.sconvGemmNCHW.argtypes = L.sconvGemmNHWC.argtypes
+    L.sconvGemmNCHW.restype = None
+    L.sconvGemmNHWC_back.argtypes = [ci] * 13 + [vp] * 5
+    L.sconvGemmNHWC_back.restype = None
+    L.sconvGemmNCHW_back.argtypes = L.sconvGemmNHWC_back.argtypes
+    L.sconvGemmNCHW_back.restype = None
+    L.convgemm_conv_ex.argtypes = [ctypes.POINTER(Desc), cf, cf, vp, vp, vp, ctypes.POINTER(Epilogue), ci, vp, vp,
+                                   ctypes.c_size_t, vp]
+    L.convgemm_conv_ex.restype = ci
+    L.convgemm_workspace_size.argtypes = [ctypes.POINTER(Desc), ci]
+    L.convgemm_workspace_size.restype = ctypes.c_size_t
+    L.convgemm_out_size.argtypes = [ci] * 5
+    L.convgemm_out_size.restype = ci
+    L.convgemm_free_pack_buffs.argtypes = [vp, vp]
+    L.convgemm_free_pack_buffs.restype = None
+    L.convgemm_set_stream.argtypes = [vp]
+    L.convgemm_set_stream.restype = ci
+    L.convgemm_get_last_error.argtypes = []
+    L.convgemm_get_last_error.restype = ci
+    L.convgemm_last_error_message.argtypes = []
+    L.convgemm_last_error_message.restype = ctypes.c_char_p
+    L.convgemm_status_string.argtypes = [ci]
+    L.convgemm_status_string.restype = ctypes.c_char_p
+    L.convgemm_version.argtypes = []
+    L.convgemm_version.restype = ci
+    L.cg_force_config.argtypes = [ci, ci]
+    L.cg_force_config.restype = None
+    L.cg_last_plan.argtypes = [ctypes.POINTER(ci)] * 3
+    L.cg_last_plan.restype = None
+    L.cg_debug_fail_next_allocs.argtypes = [ci]
+    L.cg_debug_fail_next_allocs.restype = None
+    _lib = L
+    return L
+
+
+def _ptr(t) -> Optional[int]:
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    if hasattr(t, "data_ptr"):
+        if not t.is_cuda:
+            raise ValueError("convgemm_amd operates on GPU tensors only")
+        if not t.is_contiguous():
+            raise ValueError("tensors must be contiguous")
+        return t.data_ptr()
+    raise TypeError(f"expected a torch tensor or a device address, got {type(t)}")
+
+
+def _check(L=None):
+    L = L or lib()
+    st = L.convgemm_get_last_error()
+    if st != 0:
+        raise ConvGemmError(st, L.convgemm_last_error_message().decode())
+
+
+def last_error():
+    L = lib()
+    return L.convgemm_get_last_error(), L.convgemm_last_error_message().decode()
+
+
+def out_size(inp, pad, dil, k, stride) -> int:
+    """(in + 2*pad - dil*(k-1) - 1)/stride + 1 with C division (src/convGemm.c:60-61)."""
+    return lib().convgemm_out_size(inp, pad, dil, k, stride)
+
+
+def alloc_pack_buffs():
+    """Mirror of alloc_pack_buffs (src/convGemm.c:30-40): returns (Ac_pack, Bc_pack) device addresses."""
+    L = lib()
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if L.alloc_pack_buffs(ctypes.byref(a), ctypes.byref(b)) != 0:
+        raise ConvGemmError(2, L.convgemm_last_error_message().decode())
+    return a.value, b.value
+
+
+def free_pack_buffs(ac, bc) -> None:
+    lib().convgemm_free_pack_buffs(ac, bc)
+
+
+def sconvGemmNHWC(trans, b, h, w, c, kn, kh, kw, vpadding, hpadding, vstride, hstride, vdilation, hdilation,
+                  in_, x, out, bias_vector=None, bn_running_mean=None, bn_inv_std=None, bn_gamma=None,
+                  bn_beta=None, relu=False, ac_pack=None, bc_pack=None):
+    """trans=='N': out = epi(conv(x, in_)); else out = dW = im2row(x)^T in_ (src/convGemm.c:48-77)."""
+    L = lib()
+    L.sconvGemmNHWC(trans.encode() if isinstance(trans, str) else trans, b, h, w, c, kn, kh, kw, vpadding,
+                    hpadding, vstride, hstride, vdilation, hdilation, _ptr(in_), _ptr(x), _ptr(out),
+                    _ptr(bias_vector), _ptr(bn_running_mean), _ptr(bn_inv_std), _ptr(bn_gamma), _ptr(bn_beta),
+                    bool(relu), _ptr(ac_pack), _ptr(bc_pack))
+    _check(L)
+
+
+def sconvGemmNHWC_back(b, h, w, c, kn, kh, kw, vstride, hstride, vpadding, hpadding, vdilation, hdilation,
+                       weights, dy, dx, ac_pack=None, bc_pack=None):
+    """dx += row2im(weights^T dy) (src/convGemm.c:82-97).  Stride before padding, as in the reference."""
+    L = lib()
+    L.sconvGemmNHWC_back(b, h, w, c, kn, kh, kw, vstride, hstride, vpadding, hpadding, vdilation, hdilation,
+                         _ptr(weights), _ptr(dy), _ptr(dx), _ptr(ac_pack), _ptr(bc_pack))
+    _check(L)
+
+
+def sconvGemmNCHW(trans, b, c, h, w, kn, kh, kw, vpadding, hpadding, vstride, hstride, vdilation, hdilation,
+                  in_, x, out, bias_vector=None, bn_running_mean=None, bn_inv_std=None, bn_gamma=None,
+                  bn_beta=None, relu=False, ac_pack=None, bc_pack=None):
+    """NCHW forward / weight gradient (src/convGemm.c:99-125)."""
+    L = lib()
+    L.sconvGemmNCHW(trans.encode() if isinstance(trans, str) else trans, b, c, h, w, kn, kh, kw, vpadding,
+                    hpadding, vstride, hstride, vdilation, hdilation, _ptr(in_), _ptr(x), _ptr(out),
+                    _ptr(bias_vector), _ptr(bn_running_mean), _ptr(bn_inv_std), _ptr(bn_gamma), _ptr(bn_beta),
+                    bool(relu), _ptr(ac_pack), _ptr(bc_pack))
+    _check(L)
+
+
+def sconvGemmNCHW_back(b, c, h, w, kn, kh, kw, vstride, hstride, vpadding, hpadding, vdilation, hdilation,
+                       weights, dy, dx, ac_pack=None, bc_pack=None):
+    """dx += col2im(dy weights) (src/convGemm.c:130-146)."""
+    L = lib()
+    L.sconvGemmNCHW_back(b, c, h, w, kn, kh, kw, vstride, hstride, vpadding, hpadding, vdilation, hdilation,
+                         _ptr(weights), _ptr(dy), _ptr(dx), _ptr(ac_pack), _ptr(bc_pack))
+    _check(L)
+
+
+def make_desc(op, layout, b, h, w, c, kn, kh, kw, vpad=0, hpad=0, vstride=1, hstride=1, vdil=1, hdil=1) -> Desc:
+    return Desc(op, layout, b, h, w, c, kn, kh, kw, vpad, hpad, vstride, hstride, vdil, hdil)
+
+
+def conv_ex(desc: Desc, in_, x, out, alpha=1.0, beta=0.0, bias=None, mean=None, inv_std=None, gamma=None,
+            bnbeta=None, relu=False, dtype=F32, scales=None, workspace=None, ws_bytes=0, stream=None):
+    """convgemm_conv_ex: asynchronous on `stream` (torch stream, raw handle or None = null stream)."""
+    L = lib()
+    epi = Epilogue(_ptr(bias), _ptr(mean), _ptr(inv_std), _ptr(gamma), _ptr(bnbeta), int(bool(relu)))
+    if stream is None:
+        sh = None
+    elif isinstance(stream, int):
+        sh = stream
+    else:
+        sh = stream.cuda_stream
+    st = L.convgemm_conv_ex(ctypes.byref(desc), float(alpha), float(beta), _ptr(in_), _ptr(x), _ptr(out),
+                            ctypes.byref(epi), int(dtype), _ptr(scales), _ptr(workspace), int(ws_bytes), sh)
+    if st != 0:
+        raise ConvGemmError(st, L.convgemm_last_error_message().decode())
+
+
+def workspace_size(desc: Desc, dtype=F32) -> int:
+    return int(lib().convgemm_workspace_size(ctypes.byref(desc), int(dtype)))
+
+
+def last_plan():
+    """(tile configuration, split-K factor, kernels launched) of the last call on this thread."""
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    lib().cg_last_plan(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    return a.value, b.value, c.value
+
+
+def force_config(cfg: int = -1, splits: int = 0) -> None:
+    """Tuning hook: force tile configuration `cfg` and split-K `splits` on every call (cfg < 0 clears)."""
+    lib().cg_force_config(int(cfg), int(splits))

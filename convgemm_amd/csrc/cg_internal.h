@@ -1,0 +1,87 @@
+/*
+ * cg_internal.h -- the contract between the C host layer (host/convgemm.c)
+ * and the HIP launchers (kernels/*.hip).  Plain C: no HIP or C++ types.
+ *
+ * The six convolution operations are all expressed as one implicit GEMM
+ * C[M][N] = sum_k A[M][k] * B[k][N] whose operands are gathered from the
+ * tensors on the fly:
+ *
+ *   op           M (rows)                 N        K (reduction)
+ *   FWD          output pixel p           kn       (tap, c)
+ *   DGRAD        input pixel of a stride  c        (valid tap, kn)
+ *                residue class (ih%vs, iw%hs)
+ *   WGRAD        (tap, c)                 kn       output pixel p
+ *
+ * tap = kx*kw + ky.  A reduction "chunk" is BK consecutive channels of one
+ * tap (zero-masked past the channel count), so every K tile has a fixed tap.
+ */
+#ifndef CONVGEMM_AMD_CG_INTERNAL_H
+#define CONVGEMM_AMD_CG_INTERNAL_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum cg_opkind {
+    CG_FWD_NHWC = 0,
+    CG_FWD_NCHW = 1,
+    CG_WGRAD_NHWC = 2,
+    CG_WGRAD_NCHW = 3,
+    CG_DGRAD_NHWC = 4,
+    CG_DGRAD_NCHW = 5,
+    CG_NUM_OPKINDS = 6
+};
+
+enum cg_elem { CG_ELEM_F32 = 0, CG_ELEM_BF16 = 1, CG_ELEM_FP8 = 2 };
+
+typedef struct {
+    int op; /* enum cg_opkind */
+    int B, H, W, C, KN, KH, KW;
+    int vp, hp, vs, hs, vd, hd;
+    int Ho, Wo, taps;
+    /* GEMM tile space (filled by the host from the chosen configuration) */
+    int M, N;        /* logical rows / cols (DGRAD: rows of the largest class) */
+    int mtiles, ntiles;
+    int kchunks;     /* FWD/DGRAD: ceil(reduction channels / BK); WGRAD: ceil(C / BM) */
+    int KT;          /* K tiles of one output tile (DGRAD: upper bound, all taps) */
+    int splits;      /* split-K factor (1: no partial buffer) */
+    int ncls_h, ncls_w; /* DGRAD residue classes (vs, hs); 1, 1 otherwise */
+    int vecA, vecB;  /* 16-byte vector loads allowed on the contiguous dimension */
+    long long P;     /* B*Ho*Wo */
+    const void *A;   /* FWD/WGRAD: X;  DGRAD: dY */
+    const void *Bm;  /* FWD/DGRAD: weights in [c][kh][kw][kn] order; WGRAD: dY */
+    float *out;
+    float *ws;       /* split-K partials: [splits][classes][M][N] */
+    const float *bias, *mean, *inv_std, *gamma, *bnb;
+    int relu;
+    float alpha, beta;
+    float scale;     /* product of operand scales (FP8), 1 otherwise */
+} cg_kparams;
+
+/* Tile configurations of the implicit-GEMM kernels. */
+typedef struct {
+    int BM, BN, BK;
+    int WM, WN;      /* waves along M and N (WM*WN = 4) */
+} cg_tilecfg;
+
+int cg_num_tilecfgs(int elem);
+int cg_get_tilecfg(int elem, int cfg, cg_tilecfg *out);
+/* Can configuration `cfg` run operation `op`? (loader constraints) */
+int cg_tilecfg_supports(int elem, int cfg, int op);
+
+/* Launch the implicit GEMM (grid from kp); `stream` is a hipStream_t.
+ * Returns 0 or a hipError_t. */
+int cg_launch_igemm(int elem, int cfg, const cg_kparams *kp, void *stream);
+/* Sum the split-K partials and apply the epilogue / layout store. */
+int cg_launch_splitk_reduce(const cg_kparams *kp, void *stream);
+/* out[i] = beta*out[i] (0 if beta == 0), n elements. */
+int cg_launch_scale(float *out, long long n, float beta, void *stream);
+/* dst[c][r] = src[r][c] for an R x C row-major float matrix (any element type
+ * size 4).  Used to bring NCHW weights [kn][K] into [K][kn] order. */
+int cg_launch_transpose_f32(const float *src, float *dst, int R, int C, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,120 @@
+// igemm_common.hpp -- device helpers shared by the implicit-GEMM kernels and
+// the split-K reduction: fast integer division, the per-block problem
+// context, and the output store with the fused epilogue.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../cg_internal.h"
+
+namespace cg {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// Division by a run-time constant d >= 1 for 0 <= n < 2^31
+// (Granlund-Montgomery round-up method): q = (n * m) >> s with
+// s = 31 + ceil(log2 d), m = ceil(2^s / d) < 2^32.
+struct FastDiv {
+    uint32_t m;
+    uint32_t s;
+    int d;
+};
+
+__device__ __host__ inline FastDiv make_fastdiv(int d) {
+    uint32_t l = 0;
+    while ((1u << l) < (uint32_t)d) l++;
+    uint32_t s = 31 + l;
+    uint64_t m = ((1ull << s) + (uint64_t)d - 1) / (uint64_t)d;
+    FastDiv f;
+    f.m = (uint32_t)m;
+    f.s = s;
+    f.d = d;
+    return f;
+}
+
+__device__ __host__ inline int fdiv(int n, const FastDiv &f) {
+    return (int)(((uint64_t)(uint32_t)n * f.m) >> f.s);
+}
+
+// Problem facts a block needs, computed once in the prologue.
+struct Ctx {
+    FastDiv howo, wo, cls_hw, cls_w, c;
+    int rh, rw;      // DGRAD residue class
+    int Hc, Wc;      // DGRAD class extents
+    int Mrows;       // valid logical rows of this block's problem/class
+};
+
+__device__ inline Ctx make_ctx(const cg_kparams &p, int cls) {
+    Ctx x;
+    int op = p.op;
+    x.howo = make_fastdiv(p.Ho * p.Wo > 0 ? p.Ho * p.Wo : 1);
+    x.wo = make_fastdiv(p.Wo > 0 ? p.Wo : 1);
+    x.c = make_fastdiv(p.C > 0 ? p.C : 1);
+    x.rh = 0;
+    x.rw = 0;
+    x.Hc = p.Ho;
+    x.Wc = p.Wo;
+    if (op == CG_DGRAD_NHWC || op == CG_DGRAD_NCHW) {
+        x.rh = cls / p.ncls_w;
+        x.rw = cls - x.rh * p.ncls_w;
+        x.Hc = (p.H - x.rh + p.vs - 1) / p.vs;
+        x.Wc = (p.W - x.rw + p.hs - 1) / p.hs;
+        x.Mrows = p.B * x.Hc * x.Wc;
+    } else if (op == CG_WGRAD_NHWC || op == CG_WGRAD_NCHW) {
+        x.Mrows = p.taps * p.C;
+    } else {
+        x.Mrows = (int)p.P;
+    }
+    x.cls_hw = make_fastdiv(x.Hc * x.Wc > 0 ? x.Hc * x.Wc : 1);
+    x.cls_w = make_fastdiv(x.Wc > 0 ? x.Wc : 1);
+    return x;
+}
+
+// Write one finished accumulator to the output tensor, applying
+// alpha/beta and (forward only) bias + BN + ReLU.
+//   m: logical GEMM row (FWD: pixel; DGRAD: class pixel; WGRAD: tap*C + c)
+//   n: logical GEMM col (FWD/WGRAD: kn; DGRAD: c)
+// Epilogue order follows src/im2row_nhwc.c:276-282 and
+// src/im2col_nchw.c:422-431.
+template <int OP>
+__device__ inline void store_out(const cg_kparams &p, const Ctx &x, int m, int n, float acc) {
+    long long addr;
+    if constexpr (OP == CG_FWD_NHWC) {
+        addr = (long long)m * p.KN + n;
+    } else if constexpr (OP == CG_FWD_NCHW) {
+        int b = fdiv(m, x.howo);
+        int pix = m - b * (p.Ho * p.Wo);
+        addr = ((long long)b * p.KN + n) * (p.Ho * p.Wo) + pix;
+    } else if constexpr (OP == CG_DGRAD_NHWC || OP == CG_DGRAD_NCHW) {
+        int b = fdiv(m, x.cls_hw);
+        int rem = m - b * (x.Hc * x.Wc);
+        int i = fdiv(rem, x.cls_w);
+        int j = rem - i * x.Wc;
+        int ih = x.rh + p.vs * i, iw = x.rw + p.hs * j;
+        if constexpr (OP == CG_DGRAD_NHWC)
+            addr = (((long long)b * p.H + ih) * p.W + iw) * p.C + n;
+        else
+            addr = (((long long)b * p.C + n) * p.H + ih) * p.W + iw;
+    } else {  // WGRAD
+        int tap = fdiv(m, x.c);
+        int c = m - tap * p.C;
+        if constexpr (OP == CG_WGRAD_NHWC)
+            addr = ((long long)c * p.taps + tap) * p.KN + n;
+        else
+            addr = (long long)n * ((long long)p.C * p.taps) + (long long)c * p.taps + tap;
+    }
+    float t = p.alpha * acc;
+    if (p.beta != 0.0f) t += p.beta * p.out[addr];
+    if constexpr (OP == CG_FWD_NHWC || OP == CG_FWD_NCHW) {
+        if (p.bias) t += p.bias[n];
+        if (p.mean) {
+            t = (t - p.mean[n]) * p.inv_std[n];
+            t = t * p.gamma[n] + p.bnb[n];
+        }
+        if (p.relu && t < 0.0f) t = 0.0f;
+    }
+    p.out[addr] = t;
+}
+
+}  // namespace cg

@@ -1,0 +1,177 @@
+"""C-ABI boundary checks that need no GPU.
+
+* libconvgemm.so loads and exports every function include/*.h declares;
+* the host layer rejects invalid problems before any device work, recording
+  a status (the reference has no validation: src/convGemm.c:48-146);
+* the fast-division helper used for pixel/tap decomposition in the kernels is
+  exact (checked on the host, exhaustively over small divisors).
+"""
+import ctypes
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+import convgemm_amd as cg
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    names = set()
+    for fn in ("convgemm.h", "convGemm.h"):
+        txt = open(os.path.join(ROOT, "include", fn)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt):
+            name = m.group(1)
+            if name in ("if", "sizeof", "defined"):
+                continue
+            # a declaration: preceded by a return type on the same statement
+            names.add(name)
+    return names
+
+
+def test_header_symbols_match_exported_list():
+    declared = header_functions()
+    assert declared == set(cg.EXPORTED_SYMBOLS), declared ^ set(cg.EXPORTED_SYMBOLS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = cg.lib()
+    for name in cg.EXPORTED_SYMBOLS:
+        assert hasattr(L, name), name
+    nm = subprocess.run(["nm", "-D", "--defined-only", cg.LIB_PATH], stdout=subprocess.PIPE, text=True).stdout
+    defined = {l.split()[-1] for l in nm.splitlines() if " T " in l}
+    missing = set(cg.EXPORTED_SYMBOLS) - defined
+    assert not missing, missing
+
+
+def test_drop_in_signatures_mirror_reference_order():
+    """The NHWC forward takes padding before stride, _back takes stride before padding
+    (src/convGemm.h:22-42); check the header text keeps that trap."""
+    h = open(os.path.join(ROOT, "include", "convgemm.h")).read()
+    fwd = h[h.index("void sconvGemmNHWC(char trans"):]
+    fwd = fwd[:fwd.index(";")]
+    assert fwd.index("vpadding") < fwd.index("vstride")
+    back = h[h.index("void sconvGemmNHWC_back("):]
+    back = back[:back.index(";")]
+    assert back.index("vstride") < back.index("vpadding")
+
+
+def test_version_and_status_strings():
+    L = cg.lib()
+    assert L.convgemm_version() >= 10000
+    assert L.convgemm_status_string(0) == b"success"
+    assert L.convgemm_status_string(4) == b"workspace too small"
+
+
+def test_out_size_formula():
+    assert cg.out_size(56, 1, 1, 3, 1) == 56
+    assert cg.out_size(56, 1, 1, 3, 2) == 28
+    assert cg.out_size(55, 0, 1, 11, 4) == 12
+    assert cg.out_size(7, 2, 2, 3, 1) == 7
+
+
+BAD = [
+    dict(vstride=0),
+    dict(hstride=-1),
+    dict(vdil=0),
+    dict(vpad=-1),
+    dict(kh=0),
+    dict(b=-1),
+    dict(h=2, w=2, kh=5, kw=5),      # empty output
+    dict(op=7),
+    dict(layout=3),
+]
+
+
+@pytest.mark.parametrize("bad", BAD)
+def test_conv_ex_rejects_invalid_descriptors(bad):
+    d = dict(op=cg.FWD, layout=cg.NHWC, b=1, h=8, w=8, c=4, kn=4, kh=3, kw=3, vpad=1, hpad=1, vstride=1,
+             hstride=1, vdil=1, hdil=1)
+    d.update(bad)
+    desc = cg.Desc(**d)
+    with pytest.raises(cg.ConvGemmError) as e:
+        cg.conv_ex(desc, 0x1000, 0x2000, 0x3000)
+    assert e.value.status in (1, 5)
+
+
+def test_drop_in_records_error_without_touching_device():
+    L = cg.lib()
+    # stride 0: rejected before any HIP call, nothing written, status recorded
+    L.sconvGemmNHWC(b"N", 1, 8, 8, 4, 4, 3, 3, 1, 1, 0, 1, 1, 1, 0x1000, 0x2000, 0x3000,
+                    None, None, None, None, None, False, None, None)
+    st, msg = cg.last_error()
+    assert st == 1 and "stride" in msg
+    L.sconvGemmNCHW_back(1, 4, 8, 8, 4, 3, 3, 1, 1, 1, 1, 1, 1, None, None, 0x3000, None, None)
+    st, msg = cg.last_error()
+    assert st == 1 and "NULL" in msg
+
+
+def test_batch_norm_requires_all_vectors():
+    desc = cg.make_desc(cg.FWD, cg.NHWC, 1, 8, 8, 4, 4, 3, 3, 1, 1)
+    with pytest.raises(cg.ConvGemmError) as e:
+        cg.conv_ex(desc, 0x1000, 0x2000, 0x3000, mean=0x4000)  # inv_std/gamma/beta missing
+    assert e.value.status == 1
+
+
+def test_quick_return_needs_no_device():
+    """m == 0 or n == 0 returns before any device work (B3A2C0_orig.c:45-47)."""
+    for kw in (dict(b=0), dict(kn=0)):
+        d = dict(op=cg.FWD, layout=cg.NHWC, b=1, h=8, w=8, c=4, kn=4, kh=3, kw=3, vpad=1, hpad=1, vstride=1,
+                 hstride=1, vdil=1, hdil=1)
+        d.update(kw)
+        cg.conv_ex(cg.Desc(**d), None, None, None)
+    # k == 0 (c == 0) with beta == 1: quick return
+    d = cg.make_desc(cg.FWD, cg.NHWC, 1, 8, 8, 0, 4, 3, 3, 1, 1)
+    cg.conv_ex(d, None, None, None, beta=1.0)
+
+
+def test_alloc_failure_returns_one():
+    L = cg.lib()
+    L.cg_debug_fail_next_allocs(1)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert L.alloc_pack_buffs(ctypes.byref(a), ctypes.byref(b)) == 1
+    assert a.value is None and b.value is None
+    assert L.convgemm_get_last_error() == 2
+
+
+FASTDIV_TEST = r"""
+#include "kernels/igemm_common.hpp"
+#include <cstdio>
+#include <cstdlib>
+int main() {
+    unsigned long long checked = 0;
+    for (int d = 1; d <= 4096; d++) {
+        cg::FastDiv f = cg::make_fastdiv(d);
+        for (int n = 0; n < 70000; n++) { if (cg::fdiv(n, f) != n / d) { printf("FAIL %d %d\n", n, d); return 1; } checked++; }
+        int big[] = {2147483647, 2147483646, 2147483647 - d, 1 << 30, (1 << 30) + d * 7, 123456789};
+        for (int n : big) { if (n < 0) continue; if (cg::fdiv(n, f) != n / d) { printf("FAIL %d %d\n", n, d); return 1; } }
+    }
+    int ds[] = {3136, 12544, 100352, 784, 49, 9, 25, 121, 2147483647, 1 << 30, 1000003};
+    for (int d : ds) {
+        cg::FastDiv f = cg::make_fastdiv(d);
+        for (long long n = 0; n < 2147483648LL; n += 9973) { if (cg::fdiv((int)n, f) != (int)(n / d)) { printf("FAIL %lld %d\n", n, d); return 1; } }
+    }
+    printf("OK %llu\n", checked);
+    return 0;
+}
+"""
+
+
+def test_fastdiv_exact():
+    hipcc = "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    with tempfile.TemporaryDirectory() as td:
+        src = os.path.join(td, "t.cpp")
+        exe = os.path.join(td, "t")
+        open(src, "w").write(FASTDIV_TEST)
+        r = subprocess.run([hipcc, "-O2", "-x", "hip", "--offload-arch=gfx950", "-I",
+                            os.path.join(ROOT, "convgemm_amd", "csrc"), "-I", os.path.join(ROOT, "include"),
+                            src, "-o", exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        assert r.returncode == 0, r.stdout
+        out = subprocess.run([exe], stdout=subprocess.PIPE, text=True)
+        assert out.returncode == 0 and out.stdout.startswith("OK"), out.stdout

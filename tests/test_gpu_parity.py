@@ -1,0 +1,286 @@
+"""GPU parity of the FP32 implicit-GEMM path against the oracle.
+
+Every test calls the HIP kernels through the C ABI (libconvgemm.so) and
+compares with the FP64 direct-convolution oracle (pinned in test_oracle.py).
+FP32 bar: max|y - y64| / max|y64| <= 1e-5 (normwise-max; the reference's own
+element-wise check is used on its own U[0,1] data in
+test_reference_acceptance_uniform_data).
+
+Edge cases follow what the reference's tests and the survey of its code call
+for: every epilogue combination including NULL pointers, alpha/beta,
+dilation, asymmetric stride and padding, multi-block K, channel counts that
+break 16-byte vectors, kernels larger than the stride, dX accumulation,
+empty problems, and every tile configuration with and without split-K.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from oracle import NCHW, NHWC, Shape
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    assert torch.cuda.is_available(), "GPU tests need an MI355X"
+    import convgemm_amd as cg
+    cg.lib()  # fail loudly if the HIP library is missing
+    yield
+    cg.force_config(-1, 0)
+
+
+from tests.gpu_util import (DEV, dev, host, random_problem, rel_err, run_dgrad, run_fwd,  # noqa: E402
+                            run_wgrad)
+import convgemm_amd as cg  # noqa: E402
+
+SHAPES = [
+    Shape(1, 56, 56, 64, 64, 3, 3, 1, 1, 1, 1, 1, 1),   # the reference's CPU configuration (batch 1)
+    Shape(2, 9, 7, 5, 6, 3, 3, 1, 1, 1, 1, 1, 1),       # C, kn not multiples of 4
+    Shape(2, 10, 11, 8, 12, 3, 3, 2, 1, 1, 2, 2, 1),    # dilation, asymmetric stride and padding
+    Shape(3, 8, 8, 16, 20, 1, 1, 0, 0, 1, 1, 1, 1),     # 1x1
+    Shape(2, 15, 13, 12, 16, 5, 5, 2, 2, 2, 2, 1, 1),   # 5x5 s2
+    Shape(1, 23, 23, 3, 16, 11, 11, 0, 0, 4, 4, 1, 1),  # AlexNet-like first layer, C = 3
+    Shape(2, 14, 14, 40, 24, 3, 3, 1, 1, 3, 3, 1, 1),   # stride 3 > half kernel
+    Shape(1, 6, 6, 4, 4, 6, 6, 0, 0, 1, 1, 1, 1),       # full window
+    Shape(2, 7, 7, 100, 36, 3, 3, 1, 1, 1, 1, 1, 1),    # K = 900, several K tiles, ragged
+    Shape(1, 12, 9, 8, 8, 2, 3, 0, 1, 2, 3, 3, 2),      # even kernel, stride 3, dilation 3
+    Shape(1, 1, 1, 7, 5, 1, 1, 0, 0, 1, 1, 1, 1),       # single pixel
+    Shape(2, 16, 16, 32, 32, 7, 7, 3, 3, 2, 2, 1, 1),   # 7x7 s2 p3
+]
+
+
+def _case(rng, s, layout):
+    print(f"case layout={layout} {s}", flush=True)
+    return random_problem(rng, s, layout)
+
+
+@pytest.mark.parametrize("layout", [NHWC, NCHW])
+@pytest.mark.parametrize("si", range(len(SHAPES)))
+def test_forward_epilogue(layout, si):
+    s = SHAPES[si]
+    rng = np.random.default_rng(1000 + si)
+    X, W, _, epi = _case(rng, s, layout)
+    Y = run_fwd(layout, s, W, X, epi)
+    ref = oracle.fwd(layout, s, W, X, **epi)
+    mx, nrm = rel_err(Y, ref)
+    assert mx <= TOL, (mx, nrm)
+
+
+@pytest.mark.parametrize("layout", [NHWC, NCHW])
+@pytest.mark.parametrize("si", range(len(SHAPES)))
+def test_weight_gradient(layout, si):
+    s = SHAPES[si]
+    rng = np.random.default_rng(2000 + si)
+    X, _, dY, _ = _case(rng, s, layout)
+    dW = run_wgrad(layout, s, dY, X)
+    mx, nrm = rel_err(dW, oracle.wgrad(layout, s, dY, X))
+    assert mx <= TOL, (mx, nrm)
+
+
+@pytest.mark.parametrize("layout", [NHWC, NCHW])
+@pytest.mark.parametrize("si", range(len(SHAPES)))
+def test_backward_data_accumulates(layout, si):
+    s = SHAPES[si]
+    rng = np.random.default_rng(3000 + si)
+    _, W, dY, _ = _case(rng, s, layout)
+    dX0 = rng.standard_normal(s.x_shape(layout)).astype(np.float32)
+    dX = run_dgrad(layout, s, W, dY, dX0)
+    ref = oracle.dgrad(layout, s, W, dY, dX0=dX0)  # dx += row2im(W^T dY)
+    mx, nrm = rel_err(dX, ref)
+    assert mx <= TOL, (mx, nrm)
+
+
+CFG_SHAPES = [Shape(2, 11, 10, 24, 40, 3, 3, 1, 1, 1, 1, 1, 1), Shape(1, 13, 12, 20, 36, 3, 3, 1, 0, 2, 2, 1, 1)]
+
+
+@pytest.mark.parametrize("splits", [1, 3])
+@pytest.mark.parametrize("cfg", range(9))
+def test_every_tile_configuration(cfg, splits):
+    """Force each kernel instantiation (and the split-K reduction) on every op."""
+    cg.force_config(cfg, splits)
+    try:
+        for si, s in enumerate(CFG_SHAPES):
+            for layout in (NHWC, NCHW):
+                rng = np.random.default_rng(cfg * 100 + splits * 10 + si)
+                X, W, dY, epi = _case(rng, s, layout)
+                mx, _ = rel_err(run_fwd(layout, s, W, X, epi), oracle.fwd(layout, s, W, X, **epi))
+                assert mx <= TOL, ("fwd", layout, si, mx)
+                mx, _ = rel_err(run_wgrad(layout, s, dY, X), oracle.wgrad(layout, s, dY, X))
+                assert mx <= TOL, ("wgrad", layout, si, mx)
+                dX0 = np.zeros(s.x_shape(layout), np.float32)
+                mx, _ = rel_err(run_dgrad(layout, s, W, dY, dX0), oracle.dgrad(layout, s, W, dY))
+                assert mx <= TOL, ("dgrad", layout, si, mx)
+    finally:
+        cg.force_config(-1, 0)
+
+
+@pytest.mark.parametrize("layout", [NHWC, NCHW])
+@pytest.mark.parametrize("combo", range(8))
+def test_epilogue_null_combinations(layout, combo):
+    """Any subset of bias / BN / ReLU; NULL means absent (a superset of the
+    reference, whose NHWC path dereferences a NULL bias: im2row_nhwc.c:362-371)."""
+    s = Shape(2, 9, 9, 8, 12, 3, 3, 1, 1, 1, 1, 1, 1)
+    rng = np.random.default_rng(50 + combo)
+    X, W, _, epi = _case(rng, s, layout)
+    use_bias, use_bn, relu = combo & 1, combo & 2, combo & 4
+    e = dict(bias=epi["bias"] if use_bias else None, relu=bool(relu))
+    if use_bn:
+        e.update(mean=epi["mean"], inv_std=epi["inv_std"], gamma=epi["gamma"], bnbeta=epi["bnbeta"])
+    mx, _ = rel_err(run_fwd(layout, s, W, X, e), oracle.fwd(layout, s, W, X, **e))
+    assert mx <= TOL
+
+
+@pytest.mark.parametrize("layout", [NHWC, NCHW])
+def test_alpha_beta_extended_api(layout):
+    s = Shape(2, 10, 9, 12, 16, 3, 3, 1, 1, 1, 1, 1, 1)
+    rng = np.random.default_rng(77)
+    X, W, dY, epi = _case(rng, s, layout)
+    # forward: Y = epi(alpha*conv + beta*Y0)
+    Y0 = rng.standard_normal(s.y_shape(layout)).astype(np.float32)
+    Yd = dev(Y0)
+    d = cg.make_desc(cg.FWD, layout, s.b, s.h, s.w, s.c, s.kn, s.kh, s.kw, s.vpad, s.hpad, s.vstr, s.hstr)
+    cg.conv_ex(d, dev(W), dev(X), Yd, alpha=-0.5, beta=1.5, bias=dev(epi["bias"]), relu=True)
+    ref = oracle.fwd(layout, s, W, X, alpha=-0.5, beta=1.5, Y0=Y0, bias=epi["bias"], relu=True)
+    assert rel_err(host(Yd), ref)[0] <= TOL
+    # weight gradient with beta
+    W0 = rng.standard_normal(s.w_shape(layout)).astype(np.float32)
+    Wd = dev(W0)
+    d = cg.make_desc(cg.WGRAD, layout, s.b, s.h, s.w, s.c, s.kn, s.kh, s.kw, s.vpad, s.hpad, s.vstr, s.hstr)
+    cg.conv_ex(d, dev(dY), dev(X), Wd, alpha=2.0, beta=-1.0)
+    assert rel_err(host(Wd), oracle.wgrad(layout, s, dY, X, alpha=2.0, beta=-1.0, dW0=W0))[0] <= TOL
+    # backward data with beta = 0 (overwrite) and beta = 0.25
+    for beta in (0.0, 0.25):
+        X0 = rng.standard_normal(s.x_shape(layout)).astype(np.float32)
+        Xd = dev(X0)
+        d = cg.make_desc(cg.DGRAD, layout, s.b, s.h, s.w, s.c, s.kn, s.kh, s.kw, s.vpad, s.hpad, s.vstr, s.hstr)
+        cg.conv_ex(d, dev(W), dev(dY), Xd, alpha=0.75, beta=beta)
+        assert rel_err(host(Xd), oracle.dgrad(layout, s, W, dY, alpha=0.75, beta=beta, dX0=X0))[0] <= TOL
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("op", [cg.FWD, cg.WGRAD, cg.DGRAD])
+def test_alpha_zero_scales_output_only(op):
+    """quick_gemm.h: alpha == 0 gives C = beta*C without the epilogue."""
+    s = Shape(1, 6, 6, 4, 8, 3, 3, 1, 1, 1, 1, 1, 1)
+    shape = {cg.FWD: s.y_shape(NHWC), cg.WGRAD: s.w_shape(NHWC), cg.DGRAD: s.x_shape(NHWC)}[op]
+    O0 = np.random.default_rng(5).standard_normal(shape).astype(np.float32)
+    Od = dev(O0)
+    d = cg.make_desc(op, NHWC, s.b, s.h, s.w, s.c, s.kn, s.kh, s.kw, 1, 1)
+    dummy = dev(np.ones(4096, np.float32))
+    cg.conv_ex(d, dummy, dummy, Od, alpha=0.0, beta=-2.0, bias=dummy, relu=True)
+    np.testing.assert_array_equal(host(Od), -2.0 * O0)
+    cg.conv_ex(d, dummy, dummy, Od, alpha=0.0, beta=0.0)
+    assert not host(Od).any()
+
+
+def test_empty_and_zero_channel_problems_leave_output_untouched():
+    Y0 = np.random.default_rng(6).standard_normal((1, 4, 4, 3)).astype(np.float32)
+    Yd = dev(Y0)
+    dummy = dev(np.ones(64, np.float32))
+    # c == 0: the reference's K loop runs zero times and C is not written
+    cg.sconvGemmNHWC("N", 1, 4, 4, 0, 3, 3, 3, 1, 1, 1, 1, 1, 1, dummy, dummy, Yd)
+    np.testing.assert_array_equal(host(Yd), Y0)
+    # b == 0: quick return
+    cg.sconvGemmNHWC("N", 0, 4, 4, 2, 3, 3, 3, 1, 1, 1, 1, 1, 1, dummy, dummy, Yd)
+    np.testing.assert_array_equal(host(Yd), Y0)
+
+
+@pytest.mark.parametrize("layout", [NHWC, NCHW])
+def test_deterministic_bitwise(layout):
+    """No atomics anywhere: two runs are bit-identical, also with split-K."""
+    s = Shape(4, 14, 14, 64, 64, 3, 3, 1, 1, 1, 1, 1, 1)
+    rng = np.random.default_rng(8)
+    X, W, dY, epi = _case(rng, s, layout)
+    for cfg, splits in ((-1, 0), (3, 4)):
+        cg.force_config(cfg, splits)
+        a = run_fwd(layout, s, W, X, epi), run_wgrad(layout, s, dY, X), run_dgrad(
+            layout, s, W, dY, np.zeros(s.x_shape(layout), np.float32))
+        b = run_fwd(layout, s, W, X, epi), run_wgrad(layout, s, dY, X), run_dgrad(
+            layout, s, W, dY, np.zeros(s.x_shape(layout), np.float32))
+        for u, v in zip(a, b):
+            np.testing.assert_array_equal(u, v)
+    cg.force_config(-1, 0)
+
+
+@pytest.mark.parametrize("layout", [NHWC, NCHW])
+def test_reference_acceptance_uniform_data(layout):
+    """The reference's own check (tests/test_base.h:12-29): U[0,1] inputs, bias+BN+ReLU
+    all on (test_base.c:13-15), element-wise |a-b|/|a| <= 1e-5."""
+    s = Shape(2, 16, 16, 32, 48, 3, 3, 1, 1, 1, 1, 1, 1)
+    rng = np.random.default_rng(9)
+    u = lambda *sh: rng.uniform(0, 1, sh).astype(np.float32)
+    X, W = u(*s.x_shape(layout)), u(*s.w_shape(layout))
+    epi = dict(bias=u(s.kn), mean=u(s.kn), inv_std=u(s.kn), gamma=u(s.kn), bnbeta=u(s.kn), relu=True)
+    Y = run_fwd(layout, s, W, X, epi).astype(np.float64)
+    ref = oracle.fwd(layout, s, W, X, **epi)
+    nz = ref != 0
+    assert np.max(np.abs((ref[nz] - Y[nz]) / ref[nz])) <= 1e-5
+    assert np.all(Y[~nz] == 0)
+
+
+def test_pack_buffers_and_caller_workspace():
+    """alloc_pack_buffs workspaces, a caller workspace, and a too-small one."""
+    s = Shape(2, 12, 12, 32, 40, 3, 3, 1, 1, 1, 1, 1, 1)
+    ac, bc = cg.alloc_pack_buffs()
+    try:
+        rng = np.random.default_rng(10)
+        for layout in (NHWC, NCHW):
+            X, W, dY, epi = _case(rng, s, layout)
+            cg.force_config(3, 4)  # split-K: partials go to Ac_pack, NCHW weights to Bc_pack
+            assert rel_err(run_fwd(layout, s, W, X, epi, packs=(ac, bc)), oracle.fwd(layout, s, W, X, **epi))[0] <= TOL
+            dX0 = np.zeros(s.x_shape(layout), np.float32)
+            assert rel_err(run_dgrad(layout, s, W, dY, dX0, packs=(ac, bc)), oracle.dgrad(layout, s, W, dY))[0] <= TOL
+            cg.force_config(-1, 0)
+        d = cg.make_desc(cg.FWD, NCHW, s.b, s.h, s.w, s.c, s.kn, s.kh, s.kw, 1, 1)
+        need = cg.workspace_size(d)
+        ws = torch.empty(need // 4 + 64, device=DEV)
+        X, W, _, epi = _case(rng, s, NCHW)
+        Yd = torch.empty(s.y_shape(NCHW), device=DEV)
+        cg.conv_ex(d, dev(W), dev(X), Yd, workspace=ws, ws_bytes=ws.numel() * 4)
+        assert rel_err(host(Yd), oracle.fwd(NCHW, s, W, X))[0] <= TOL
+        with pytest.raises(cg.ConvGemmError) as e:
+            cg.conv_ex(d, dev(W), dev(X), Yd, workspace=ws, ws_bytes=64)
+        assert e.value.status == 4
+    finally:
+        cg.free_pack_buffs(ac, bc)
+        cg.force_config(-1, 0)
+
+
+def _fixture_shapes(max_macs=6e7, limit=40):
+    """Unique CNN layer shapes from the reference's shape list (tests/test.txt,
+    columns h w c b kh kw kn stride padding), at batch 1."""
+    rows = []
+    seen = set()
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "cnn_layer_shapes.txt")
+    for line in open(path):
+        if line.startswith("#") or not line.strip():
+            continue
+        h, w, c, b, kh, kw, kn, st, pad = map(int, line.split())
+        key = (h, w, c, kh, kw, kn, st, pad)
+        if key in seen:
+            continue
+        seen.add(key)
+        s = Shape(1, h, w, c, kn, kh, kw, pad, pad, st, st, 1, 1)
+        if s.ho <= 0 or s.wo <= 0 or s.P * s.K * s.kn > max_macs:
+            continue
+        rows.append(s)
+    rows.sort(key=lambda s: s.P * s.K * s.kn)
+    step = max(1, len(rows) // limit)
+    return rows[::step][:limit]
+
+
+@pytest.mark.parametrize("si", range(len(_fixture_shapes())))
+def test_cnn_layer_shape_fixture(si):
+    s = _fixture_shapes()[si]
+    for layout in (NHWC, NCHW):
+        rng = np.random.default_rng(4000 + si)
+        X, W, dY, epi = _case(rng, s, layout)
+        assert rel_err(run_fwd(layout, s, W, X, epi), oracle.fwd(layout, s, W, X, **epi))[0] <= TOL
+        assert rel_err(run_wgrad(layout, s, dY, X), oracle.wgrad(layout, s, dY, X))[0] <= TOL
+        dX0 = np.zeros(s.x_shape(layout), np.float32)
+        assert rel_err(run_dgrad(layout, s, W, dY, dX0), oracle.dgrad(layout, s, W, dY))[0] <= TOL

@@ -278,3 +278,29 @@ def test_port_reference_acceptance_on_uniform_data(layout):
     ref = oracle.fwd(layout, s, W, X, **epi)
     nz = ref != 0
     assert np.all(np.abs((ref[nz] - Y[nz]) / ref[nz]) <= 1e-5)
+
+
+# ---------------------------------------------------------------- sampled oracle
+
+@pytest.mark.parametrize("layout", [NHWC, NCHW])
+@pytest.mark.parametrize("si", range(len(SHAPES)))
+def test_sampled_oracle_matches_direct(layout, si):
+    from oracle import sampled
+    s = SHAPES[si]
+    X = RNG.standard_normal(s.x_shape(layout))
+    W = RNG.standard_normal(s.w_shape(layout))
+    dY = RNG.standard_normal(s.y_shape(layout))
+    epi = dict(bias=RNG.standard_normal(s.kn), mean=RNG.standard_normal(s.kn), inv_std=RNG.uniform(0.5, 2, s.kn),
+               gamma=RNG.uniform(0.5, 1.5, s.kn), bnbeta=RNG.standard_normal(s.kn), relu=True)
+    Y = oracle.fwd(layout, s, W, X, **epi)
+    ys = np.stack([RNG.integers(0, n, 40) for n in (s.b, s.ho, s.wo, s.kn)], 1)
+    np.testing.assert_allclose(sampled.fwd_sampled(layout, s, W, X, ys, **epi), sampled.nhwc_index(layout, Y, ys),
+                               rtol=1e-12, atol=1e-12)
+    dW = oracle.wgrad(layout, s, dY, X)
+    ws = np.stack([RNG.integers(0, n, 40) for n in (s.c, s.kh, s.kw, s.kn)], 1)
+    np.testing.assert_allclose(sampled.wgrad_sampled(layout, s, dY, X, ws), sampled.w_index(layout, dW, ws),
+                               rtol=1e-12, atol=1e-12)
+    dX = oracle.dgrad(layout, s, W, dY, beta=0.0)
+    xs = np.stack([RNG.integers(0, n, 40) for n in (s.b, s.h, s.w, s.c)], 1)
+    np.testing.assert_allclose(sampled.dgrad_sampled(layout, s, W, dY, xs), sampled.nhwc_index(layout, dX, xs),
+                               rtol=1e-12, atol=1e-12)
