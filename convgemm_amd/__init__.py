@@ -40,8 +40,14 @@ EXPORTED_SYMBOLS = (
     "alloc_pack_buffs", "sconvGemmNHWC", "sconvGemmNHWC_back", "sconvGemmNCHW", "sconvGemmNCHW_back",
     "convgemm_conv_ex", "convgemm_workspace_size", "convgemm_out_size", "convgemm_free_pack_buffs",
     "convgemm_set_stream", "convgemm_get_last_error", "convgemm_last_error_message", "convgemm_status_string",
-    "convgemm_version",
+    "convgemm_version", "convgemm_quantize",
+    # convgemm_tune.h
+    "convgemm_set_planner", "convgemm_autotune", "convgemm_tune_count", "convgemm_tune_save", "convgemm_tune_load",
+    "convgemm_tune_clear", "convgemm_plan", "convgemm_costmodel_train", "convgemm_costmodel_save",
+    "convgemm_costmodel_load", "convgemm_costmodel_predict",
 )
+
+PLAN_ANALYTIC, PLAN_TUNED, PLAN_LEARNED, PLAN_MODEL = 0, 1, 2, 3
 
 
 class ConvGemmError(RuntimeError):
@@ -106,6 +112,23 @@ def lib() -> ctypes.CDLL:
     L.convgemm_status_string.restype = ctypes.c_char_p
     L.convgemm_version.argtypes = []
     L.convgemm_version.restype = ci
+    L.convgemm_quantize.argtypes = [vp, vp, ctypes.c_size_t, ci, ctypes.POINTER(cf), vp]
+    L.convgemm_quantize.restype = ci
+    L.convgemm_set_planner.argtypes = [ci]
+    L.convgemm_set_planner.restype = None
+    L.convgemm_autotune.argtypes = [ctypes.POINTER(Desc), ci, vp, vp, vp, vp, ci, ctypes.POINTER(ci),
+                                    ctypes.POINTER(ci), ctypes.POINTER(cf), vp]
+    L.convgemm_autotune.restype = ci
+    L.convgemm_tune_count.argtypes = []
+    L.convgemm_tune_count.restype = ci
+    L.convgemm_tune_save.argtypes = [ctypes.c_char_p]
+    L.convgemm_tune_save.restype = ci
+    L.convgemm_tune_load.argtypes = [ctypes.c_char_p]
+    L.convgemm_tune_load.restype = ci
+    L.convgemm_tune_clear.argtypes = []
+    L.convgemm_tune_clear.restype = None
+    L.convgemm_plan.argtypes = [ctypes.POINTER(Desc), ci, ctypes.POINTER(ci), ctypes.POINTER(ci)]
+    L.convgemm_plan.restype = ci
     L.cg_force_config.argtypes = [ci, ci]
     L.cg_force_config.restype = None
     L.cg_last_plan.argtypes = [ctypes.POINTER(ci)] * 3
@@ -124,6 +147,8 @@ def _ptr(t) -> Optional[int]:
     if hasattr(t, "data_ptr"):
         if not t.is_cuda:
             raise ValueError("convgemm_amd operates on GPU tensors only")
+        if t.numel() == 0:
+            return None
         if not t.is_contiguous():
             raise ValueError("tensors must be contiguous")
         return t.data_ptr()
@@ -217,14 +242,84 @@ def conv_ex(desc: Desc, in_, x, out, alpha=1.0, beta=0.0, bias=None, mean=None, 
         sh = stream
     else:
         sh = stream.cuda_stream
+    sc = None
+    if scales is not None:  # host floats {scale_in, scale_x}
+        sc = ctypes.cast((ctypes.c_float * 2)(*[float(v) for v in scales]), ctypes.c_void_p)
     st = L.convgemm_conv_ex(ctypes.byref(desc), float(alpha), float(beta), _ptr(in_), _ptr(x), _ptr(out),
-                            ctypes.byref(epi), int(dtype), _ptr(scales), _ptr(workspace), int(ws_bytes), sh)
+                            ctypes.byref(epi), int(dtype), sc, _ptr(workspace), int(ws_bytes), sh)
     if st != 0:
         raise ConvGemmError(st, L.convgemm_last_error_message().decode())
 
 
+def quantize(src, dst, dtype, stream=None) -> float:
+    """FP32 -> BF16 (returns 1.0) or FP8 e4m3 with a per-tensor amax scale (returns the scale)."""
+    L = lib()
+    sc = ctypes.c_float(1.0)
+    sh = None if stream is None else (stream if isinstance(stream, int) else stream.cuda_stream)
+    st = L.convgemm_quantize(_ptr(src), _ptr(dst), int(src.numel()), int(dtype), ctypes.byref(sc), sh)
+    if st != 0:
+        raise ConvGemmError(st, L.convgemm_last_error_message().decode())
+    return sc.value
+
+
 def workspace_size(desc: Desc, dtype=F32) -> int:
     return int(lib().convgemm_workspace_size(ctypes.byref(desc), int(dtype)))
+
+
+def _stream_handle(stream):
+    if stream is None:
+        return None
+    return stream if isinstance(stream, int) else stream.cuda_stream
+
+
+def _scales_ptr(scales):
+    if scales is None:
+        return None
+    return ctypes.cast((ctypes.c_float * 2)(*[float(v) for v in scales]), ctypes.c_void_p)
+
+
+def set_planner(mode: int) -> None:
+    """PLAN_ANALYTIC, PLAN_TUNED (default), PLAN_LEARNED or PLAN_MODEL."""
+    lib().convgemm_set_planner(int(mode))
+
+
+def autotune(desc: Desc, in_, x, out, dtype=F32, scales=None, reps=3, stream=None):
+    """Time every (tile configuration, split-K) candidate on these buffers; returns (cfg, splits, us)."""
+    L = lib()
+    c, s, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+    st = L.convgemm_autotune(ctypes.byref(desc), int(dtype), _scales_ptr(scales), _ptr(in_), _ptr(x), _ptr(out),
+                             int(reps), ctypes.byref(c), ctypes.byref(s), ctypes.byref(us), _stream_handle(stream))
+    if st != 0:
+        raise ConvGemmError(st, L.convgemm_last_error_message().decode())
+    return c.value, s.value, us.value
+
+
+def plan(desc: Desc, dtype=F32):
+    """(tile configuration, split-K) the library would use for this problem now."""
+    L = lib()
+    c, s = ctypes.c_int(), ctypes.c_int()
+    st = L.convgemm_plan(ctypes.byref(desc), int(dtype), ctypes.byref(c), ctypes.byref(s))
+    if st != 0:
+        raise ConvGemmError(st, L.convgemm_last_error_message().decode())
+    return c.value, s.value
+
+
+def tune_count() -> int:
+    return int(lib().convgemm_tune_count())
+
+
+def tune_save(path: str) -> None:
+    if lib().convgemm_tune_save(path.encode()) != 0:
+        raise ConvGemmError(1, lib().convgemm_last_error_message().decode())
+
+
+def tune_load(path: str) -> None:
+    if lib().convgemm_tune_load(path.encode()) != 0:
+        raise ConvGemmError(1, lib().convgemm_last_error_message().decode())
+
+
+def tune_clear() -> None:
+    lib().convgemm_tune_clear()
 
 
 def last_plan():

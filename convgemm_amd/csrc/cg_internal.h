@@ -46,10 +46,12 @@ typedef struct {
     int KT;          /* K tiles of one output tile (DGRAD: upper bound, all taps) */
     int splits;      /* split-K factor (1: no partial buffer) */
     int ncls_h, ncls_w; /* DGRAD residue classes (vs, hs); 1, 1 otherwise */
-    int vecA, vecB;  /* 16-byte vector loads allowed on the contiguous dimension */
+    int vec;         /* reduction channels are a multiple of the 16-byte chunk (see cg_vec_ok) */
     long long P;     /* B*Ho*Wo */
     const void *A;   /* FWD/WGRAD: X;  DGRAD: dY */
-    const void *Bm;  /* FWD/DGRAD: weights in [c][kh][kw][kn] order; WGRAD: dY */
+    const void *Bm;  /* FWD F32 / DGRAD: weights in [c][kh][kw][kn] order;
+                        FWD BF16/FP8: weights in [kn][kh][kw][c] order; WGRAD: dY */
+    long long a_bytes, b_bytes; /* extents of A and Bm (each < 2 GiB) */
     float *out;
     float *ws;       /* split-K partials: [splits][classes][M][N] */
     const float *bias, *mean, *inv_std, *gamma, *bnb;
@@ -66,8 +68,8 @@ typedef struct {
 
 int cg_num_tilecfgs(int elem);
 int cg_get_tilecfg(int elem, int cfg, cg_tilecfg *out);
-/* Can configuration `cfg` run operation `op`? (loader constraints) */
-int cg_tilecfg_supports(int elem, int cfg, int op);
+/* Is (elem, cfg, op, vec) instantiated? */
+int cg_tilecfg_supports(int elem, int cfg, int op, int vec);
 
 /* Launch the implicit GEMM (grid from kp); `stream` is a hipStream_t.
  * Returns 0 or a hipError_t. */
@@ -76,9 +78,14 @@ int cg_launch_igemm(int elem, int cfg, const cg_kparams *kp, void *stream);
 int cg_launch_splitk_reduce(const cg_kparams *kp, void *stream);
 /* out[i] = beta*out[i] (0 if beta == 0), n elements. */
 int cg_launch_scale(float *out, long long n, float beta, void *stream);
-/* dst[c][r] = src[r][c] for an R x C row-major float matrix (any element type
- * size 4).  Used to bring NCHW weights [kn][K] into [K][kn] order. */
-int cg_launch_transpose_f32(const float *src, float *dst, int R, int C, void *stream);
+/* dst = src with its three dimensions [d0][d1][d2] permuted:
+ *   perm 0: [d0][d1][d2] -> [d2][d1][d0]
+ *   perm 1: [d0][d1][d2] -> [d0][d2][d1]
+ * esize: element size in bytes (4, 2 or 1). */
+/* FP32 -> BF16 (elem CG_ELEM_BF16) or FP8 e4m3fn with a per-tensor amax
+ * scale (CG_ELEM_FP8; `scratch`: 8 device bytes, the scale lands at +4). */
+int cg_launch_quantize(const float *src, void *dst, long long n, int elem, void *scratch, void *stream);
+int cg_launch_permute3(const void *src, void *dst, int d0, int d1, int d2, int perm, int esize, void *stream);
 
 #ifdef __cplusplus
 }

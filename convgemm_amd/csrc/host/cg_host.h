@@ -12,14 +12,16 @@
 /* Size of each buffer handed out by alloc_pack_buffs. */
 #define CG_PACK_BYTES (64 << 20)
 
-/* Planner hook: given the problem (tile fields unset), choose a tile
- * configuration and a split-K factor (cfg < 0: fall back to the default). */
-typedef void (*cg_plan_hook_t)(void *ctx, const cg_kparams *kp, int *cfg, int *splits);
+/* Planner hook: given the operand type and the problem (tile fields unset),
+ * choose a tile configuration and a split-K factor (cfg < 0: use the default). */
+typedef void (*cg_plan_hook_t)(void *ctx, int elem, const cg_kparams *kp, int *cfg, int *splits);
 
 void cg_set_plan_hook(cg_plan_hook_t hook, void *ctx);
 /* Force every call to use (cfg, splits); cfg < 0 clears it. Tuning hook. */
 void cg_force_config(int cfg, int splits);
 void cg_fill_tiles(cg_kparams *kp, const cg_tilecfg *t, int splits);
+/* The analytic ("no learning") planner. */
+void cg_default_plan(int elem, const cg_kparams *base, int *cfg, int *splits);
 size_t cg_partial_bytes(const cg_kparams *kp);
 /* Plan chosen by the last call on this thread: tile configuration, split-K
  * factor and number of kernels launched. */
@@ -29,10 +31,13 @@ void cg_debug_fail_next_allocs(int n);
 
 convgemm_status_t cg_set_error(convgemm_status_t st, const char *msg);
 
-/* BF16 / FP8 operand paths (lowp.c). */
-convgemm_status_t cg_run_lowp(const convgemm_desc_t *d, float alpha, float beta, const void *in, const void *x,
-                              void *out, const convgemm_epilogue_t *epi, convgemm_dtype_t dtype,
-                              const float *scales, void *stream);
-size_t cg_lowp_extra_bytes(const convgemm_desc_t *d, convgemm_dtype_t dtype);
+/* Validate a call and describe it as kernel parameters (tile fields unset,
+ * B/P of the first batch chunk), the element type and the batch chunk.
+ * wsw: the weight workspace the call will use (NULL: internal, aligned). */
+convgemm_status_t cg_prepare(const convgemm_desc_t *d, convgemm_dtype_t dtype, const void *in, const void *x,
+                             const void *wsw, cg_kparams *kp, int *elem, int *chunk);
+/* Candidate plans of a prepared problem: every supported configuration with
+ * split-K factors that leave >= 2 K tiles per split.  Returns the count. */
+int cg_candidates(int elem, const cg_kparams *kp, int *cfgs, int *splits, int max);
 
 #endif

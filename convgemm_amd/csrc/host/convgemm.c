@@ -5,9 +5,11 @@
  * pointers plus the extended API of include/convgemm.h.  All device work is
  * reached through the extern "C" launchers of cg_internal.h.
  *
- * Per call: validate -> reference quick-return rules -> plan (tile
- * configuration + split-K) -> optional weight repack -> implicit GEMM ->
- * optional split-K reduction.  Drop-in calls then wait for the stream.
+ * Per call: validate -> reference quick-return rules -> weight repack when the
+ * kernels need another weight order -> plan (tile configuration + split-K)
+ * -> implicit GEMM (per batch chunk when a tensor exceeds the 2 GiB buffer
+ * window) -> optional split-K reduction.  Drop-in calls then wait for the
+ * stream, like the blocking reference.
  */
 #include <pthread.h>
 #include <stdarg.h>
@@ -22,12 +24,14 @@
 #include "../cg_internal.h"
 #include "cg_host.h"
 
-#define CG_VERSION 10000 /* 1.0.0 */
+#define CG_VERSION 10100 /* 1.1.0 */
+#define CG_WINDOW (1ll << 31) /* bytes one buffer resource can address */
 
 /* ------------------------------------------------------------------ state */
 
 static _Thread_local convgemm_status_t t_last_status = CONVGEMM_SUCCESS;
 static _Thread_local char t_last_msg[256] = "";
+static _Thread_local int t_plan_cfg = -1, t_plan_splits = 0, t_plan_kernels = 0;
 
 static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
 static hipStream_t g_stream = NULL;
@@ -35,6 +39,8 @@ static hipStream_t g_stream = NULL;
 /* internal workspace used when the caller passes none */
 static void *g_ws = NULL;
 static size_t g_ws_bytes = 0;
+/* device scratch of the quantizer (amax bits, scale) */
+static void *g_qscratch = NULL;
 
 /* buffers handed out by alloc_pack_buffs, with their sizes */
 #define CG_MAX_PACKS 64
@@ -46,6 +52,8 @@ static int g_force_cfg = -1, g_force_splits = 0;
 /* optional planner hook installed by the autotuner / cost model */
 static cg_plan_hook_t g_plan_hook = NULL;
 static void *g_plan_hook_ctx = NULL;
+
+static int g_fail_allocs = 0;
 
 static convgemm_status_t set_error(convgemm_status_t st, const char *fmt, ...) {
     t_last_status = st;
@@ -102,17 +110,12 @@ void cg_set_plan_hook(cg_plan_hook_t hook, void *ctx) {
     g_plan_hook_ctx = ctx;
 }
 
-/* plan of the last call on this thread (introspection for tuning and benchmarks) */
-static _Thread_local int t_plan_cfg = -1, t_plan_splits = 0, t_plan_kernels = 0;
-
 void cg_last_plan(int *cfg, int *splits, int *nkernels) {
     if (cfg) *cfg = t_plan_cfg;
     if (splits) *splits = t_plan_splits;
     if (nkernels) *nkernels = t_plan_kernels;
 }
 
-/* Test-only: make the next `n` device allocations of the library fail. */
-static int g_fail_allocs = 0;
 void cg_debug_fail_next_allocs(int n) { g_fail_allocs = n; }
 
 static hipError_t lib_malloc(void **p, size_t bytes) {
@@ -124,12 +127,21 @@ static hipError_t lib_malloc(void **p, size_t bytes) {
     return hipMalloc(p, bytes);
 }
 
+static convgemm_status_t hip_fail(hipError_t e, const char *what) {
+    return set_error(CONVGEMM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
 /* ------------------------------------------------------------- shapes */
 
 int convgemm_out_size(int in, int pad, int dil, int k, int stride) {
     if (stride <= 0) return 0;
     return (in + 2 * pad - dil * (k - 1) - 1) / stride + 1;
 }
+
+static int elem_of(convgemm_dtype_t dt) {
+    return dt == CONVGEMM_BF16 ? CG_ELEM_BF16 : dt == CONVGEMM_FP8_E4M3 ? CG_ELEM_FP8 : CG_ELEM_F32;
+}
+static int esize_of(int elem) { return elem == CG_ELEM_F32 ? 4 : elem == CG_ELEM_BF16 ? 2 : 1; }
 
 static convgemm_status_t validate(const convgemm_desc_t *d, int *ho, int *wo) {
     if (!d) return set_error(CONVGEMM_ERR_INVALID_VALUE, "descriptor is NULL");
@@ -152,13 +164,14 @@ static convgemm_status_t validate(const convgemm_desc_t *d, int *ho, int *wo) {
     if (*ho <= 0 || *wo <= 0)
         return set_error(CONVGEMM_ERR_INVALID_VALUE, "empty output (ho=%d wo=%d): kernel larger than padded input",
                          *ho, *wo);
-    /* 32-bit index ranges used by the kernels */
-    long long P = (long long)d->b * *ho * *wo;
-    long long X = (long long)d->b * d->h * d->w * d->c;
     long long K = (long long)d->c * d->kh * d->kw;
-    if (P >= (1ll << 31) || X >= (1ll << 40) || K * d->kn >= (1ll << 40) || K >= (1ll << 31) ||
-        (long long)d->h * d->w >= (1ll << 31))
-        return set_error(CONVGEMM_ERR_UNSUPPORTED, "problem too large for 32-bit pixel indexing");
+    /* one image of X / Y and the whole weight tensor must fit one 2 GiB
+     * buffer window (larger batches are split into chunks) */
+    long long img_x = (long long)d->h * d->w * d->c * 4, img_y = (long long)*ho * *wo * d->kn * 4;
+    if (img_x >= CG_WINDOW || img_y >= CG_WINDOW || K * d->kn * 4 >= CG_WINDOW ||
+        (long long)d->b * *ho * *wo >= CG_WINDOW)
+        return set_error(CONVGEMM_ERR_UNSUPPORTED, "one image or the weights exceed the 2 GiB addressing window");
+    if (d->kn > 65535) return set_error(CONVGEMM_ERR_UNSUPPORTED, "kn > 65535");
     if (d->op == CONVGEMM_DGRAD && (d->kh > 64 || d->kw > 64))
         return set_error(CONVGEMM_ERR_UNSUPPORTED, "backward data supports kernels up to 64x64");
     return CONVGEMM_SUCCESS;
@@ -175,7 +188,7 @@ static size_t pack_bytes_of(const void *p) {
     pthread_mutex_unlock(&g_lock);
     /* unknown caller buffer: the reference contract says it is at least as
      * large as what alloc_pack_buffs hands out */
-    return r ? r : CG_PACK_BYTES;
+    return r ? r : (size_t)CG_PACK_BYTES;
 }
 
 static convgemm_status_t internal_ws(size_t need, void **out) {
@@ -255,7 +268,7 @@ static int opkind(const convgemm_desc_t *d) {
 
 static int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
-/* Fill the tile-space fields of kp for configuration cfg. */
+/* Fill the tile-space fields of kp for configuration t. */
 void cg_fill_tiles(cg_kparams *kp, const cg_tilecfg *t, int splits) {
     int op = kp->op;
     kp->ncls_h = 1;
@@ -294,34 +307,35 @@ size_t cg_partial_bytes(const cg_kparams *kp) {
     return (size_t)kp->splits * kp->ncls_h * kp->ncls_w * (size_t)kp->M * kp->N * sizeof(float);
 }
 
-/* Analytic default plan: fill the 256 CUs, prefer large tiles, split K when
- * the tile grid is too small to occupy the chip. */
-static void default_plan(const cg_kparams *base, int *cfg_out, int *splits_out) {
+/* Analytic default plan ("no learning"): estimate time as waves of
+ * workgroups over the 256 CUs times the per-tile MFMA work, penalise small
+ * tiles (operand reuse) and charge the split-K partial traffic; in the spirit
+ * of the reference's analytic blocking model (gemm_blis.h:121-148). */
+void cg_default_plan(int elem, const cg_kparams *base, int *cfg_out, int *splits_out) {
     const int CUS = 256;
     int op = base->op;
-    int is_wg = (op == CG_WGRAD_NHWC || op == CG_WGRAD_NCHW);
     double best = 1e300;
-    int best_cfg = 3, best_splits = 1;
-    int n = cg_num_tilecfgs(CG_ELEM_F32);
+    int best_cfg = -1, best_splits = 1;
+    int n = cg_num_tilecfgs(elem);
     for (int cfg = 0; cfg < n; cfg++) {
         cg_tilecfg t;
-        if (cg_get_tilecfg(CG_ELEM_F32, cfg, &t) || !cg_tilecfg_supports(CG_ELEM_F32, cfg, op)) continue;
-        if (is_wg != (t.BK == 32)) continue;
+        if (cg_get_tilecfg(elem, cfg, &t) || !cg_tilecfg_supports(elem, cfg, op, base->vec)) continue;
         cg_kparams kp = *base;
         cg_fill_tiles(&kp, &t, 1);
         long long tiles = (long long)kp.mtiles * kp.ntiles * kp.ncls_h * kp.ncls_w;
+        int lds = (t.BM + t.BN) * (t.BK * esize_of(elem) + 16) * 2;
+        int per_cu = 163840 / lds;
+        if (per_cu > 4) per_cu = 4;
+        if (per_cu < 1) per_cu = 1;
         for (int s = 1; s <= 32; s *= 2) {
-            if (s > 1 && kp.KT / s < 4) break;
-            /* time ~ waves of workgroups x per-tile work, with a padding cost
-             * for tile edges and a fixed cost per split-K reduction */
+            if (s > 1 && kp.KT / s < 2) break;
             long long wgs = tiles * s;
-            int per_cu = 2; /* resident workgroups per CU */
-            long long waves = (wgs + CUS * per_cu - 1) / (CUS * per_cu);
-            double kwork = (double)kp.KT / s * t.BK;
-            double tile_eff = (double)t.BM * t.BN / (t.BM + t.BN); /* reuse per loaded byte */
-            double cost = waves * (double)t.BM * t.BN * kwork * per_cu / 1.0;
-            cost *= (1.0 + 32.0 / tile_eff);
-            if (s > 1) cost += (double)kp.M * kp.N * s * 0.5 * (kp.ncls_h * kp.ncls_w);
+            long long slots = (long long)CUS * per_cu;
+            double waves = (double)((wgs + slots - 1) / slots);
+            double kwork = (double)((kp.KT + s - 1) / s) * t.BK;
+            double cost = waves * (double)t.BM * t.BN * kwork * per_cu;
+            cost *= 1.0 + 48.0 / ((double)t.BM * t.BN / (t.BM + t.BN));
+            if (s > 1) cost += 8.0 * (double)kp.M * kp.N * s * kp.ncls_h * kp.ncls_w;
             if (cost < best) {
                 best = cost;
                 best_cfg = cfg;
@@ -342,21 +356,114 @@ typedef struct {
     size_t w_bytes;
 } cg_wsets;
 
-static convgemm_status_t hip_fail(hipError_t e, const char *what) {
-    return set_error(CONVGEMM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+/* Weight order the kernels read, and the repack that produces it
+ * (arguments of cg_launch_permute3); returns 0 when none is needed. */
+static int weight_repack(const convgemm_desc_t *d, int elem, int *d0, int *d1, int *d2, int *perm) {
+    int taps = d->kh * d->kw;
+    int K = d->c * taps;
+    if (d->op == CONVGEMM_WGRAD) return 0;
+    if (d->op == CONVGEMM_FWD && elem != CG_ELEM_F32) {
+        if (d->layout == CONVGEMM_NHWC) { /* [c][tap][kn] -> [kn][tap][c] */
+            *d0 = d->c; *d1 = taps; *d2 = d->kn; *perm = 0;
+        } else { /* [kn][c][tap] -> [kn][tap][c] */
+            *d0 = d->kn; *d1 = d->c; *d2 = taps; *perm = 1;
+        }
+        return 1;
+    }
+    if (d->layout == CONVGEMM_NCHW) { /* [kn][K] -> [K][kn] */
+        *d0 = d->kn; *d1 = 1; *d2 = K; *perm = 0;
+        return 1;
+    }
+    return 0;
+}
+
+static int vec_ok(int elem, const convgemm_desc_t *d, const void *a, const void *b) {
+    int vw = 16 / esize_of(elem);
+    if (((uintptr_t)a % 16) || ((uintptr_t)b % 16)) return 0;
+    if (d->op == CONVGEMM_FWD) return d->c % vw == 0;
+    if (d->op == CONVGEMM_DGRAD) return d->kn % vw == 0;
+    return 1;
+}
+
+/* Bytes per image of the A operand, the output, and (WGRAD) the dY operand. */
+static void image_bytes(const convgemm_desc_t *d, int ho, int wo, int es, long long *img_a, long long *img_o,
+                        long long *img_b) {
+    *img_o = 0;
+    *img_b = 0;
+    if (d->op == CONVGEMM_FWD) {
+        *img_a = (long long)d->h * d->w * d->c * es;
+        *img_o = (long long)ho * wo * d->kn * 4;
+    } else if (d->op == CONVGEMM_WGRAD) {
+        *img_a = (long long)d->h * d->w * d->c * es;
+        *img_b = (long long)ho * wo * d->kn * es;
+    } else {
+        *img_a = (long long)ho * wo * d->kn * es;
+        *img_o = (long long)d->h * d->w * d->c * 4;
+    }
+}
+
+convgemm_status_t cg_prepare(const convgemm_desc_t *d, convgemm_dtype_t dtype, const void *in, const void *x,
+                             const void *wsw, cg_kparams *kp, int *elem_out, int *chunk_out) {
+    int ho, wo;
+    convgemm_status_t st = validate(d, &ho, &wo);
+    if (st != CONVGEMM_SUCCESS) return st;
+    if (dtype != CONVGEMM_F32 && dtype != CONVGEMM_BF16 && dtype != CONVGEMM_FP8_E4M3)
+        return set_error(CONVGEMM_ERR_INVALID_VALUE, "unknown dtype %d", (int)dtype);
+    const int elem = elem_of(dtype), es = esize_of(elem);
+    if (elem == CG_ELEM_FP8 && d->op != CONVGEMM_FWD)
+        return set_error(CONVGEMM_ERR_UNSUPPORTED, "FP8 operands are supported for the forward pass only");
+    memset(kp, 0, sizeof(*kp));
+    kp->op = opkind(d);
+    kp->B = d->b; kp->H = d->h; kp->W = d->w; kp->C = d->c; kp->KN = d->kn; kp->KH = d->kh; kp->KW = d->kw;
+    kp->vp = d->vpad; kp->hp = d->hpad; kp->vs = d->vstride; kp->hs = d->hstride; kp->vd = d->vdil;
+    kp->hd = d->hdil;
+    kp->Ho = ho; kp->Wo = wo; kp->taps = d->kh * d->kw;
+    kp->scale = 1.0f;
+    int rd0, rd1, rd2, rperm;
+    const int repack = weight_repack(d, elem, &rd0, &rd1, &rd2, &rperm);
+    /* an internal workspace is 256-byte aligned, so only caller buffers can break vec */
+    kp->vec = vec_ok(elem, d, x, repack ? wsw : in);
+    long long img_a, img_o, img_b;
+    image_bytes(d, ho, wo, es, &img_a, &img_o, &img_b);
+    long long img_max = img_a > img_o ? img_a : img_o;
+    if (img_b > img_max) img_max = img_b;
+    int chunk = (int)((CG_WINDOW - 1) / (img_max > 0 ? img_max : 1));
+    if (chunk > d->b) chunk = d->b;
+    if (chunk < 1) chunk = 1;
+    kp->B = chunk;
+    kp->P = (long long)chunk * ho * wo;
+    *elem_out = elem;
+    *chunk_out = chunk;
+    return CONVGEMM_SUCCESS;
+}
+
+/* Plan (cfg, splits) for a prepared problem: forced plan, planner hook, analytic model. */
+static int choose_plan(int elem, const cg_kparams *kp, int *cfg, int *splits) {
+    *cfg = -1;
+    *splits = 1;
+    if (g_force_cfg >= 0 && cg_tilecfg_supports(elem, g_force_cfg, kp->op, kp->vec)) {
+        *cfg = g_force_cfg;
+        *splits = g_force_splits > 0 ? g_force_splits : 1;
+    } else if (g_plan_hook) {
+        g_plan_hook(g_plan_hook_ctx, elem, kp, cfg, splits);
+    }
+    if (*cfg < 0 || !cg_tilecfg_supports(elem, *cfg, kp->op, kp->vec)) cg_default_plan(elem, kp, cfg, splits);
+    return *cfg;
 }
 
 static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, const void *in,
                              const void *x, void *out, const convgemm_epilogue_t *epi,
                              convgemm_dtype_t dtype, const float *scales, cg_wsets *ws,
                              hipStream_t stream) {
-    int ho, wo;
-    convgemm_status_t st = validate(d, &ho, &wo);
+    cg_kparams kp;
+    int elem, chunk;
+    convgemm_status_t st = cg_prepare(d, dtype, in, x, ws->ws_w, &kp, &elem, &chunk);
     if (st != CONVGEMM_SUCCESS) return st;
-    if (dtype != CONVGEMM_F32) return cg_run_lowp(d, alpha, beta, in, x, out, epi, dtype, scales, stream);
+    const int es = esize_of(elem);
+    const int ho = kp.Ho, wo = kp.Wo;
 
-    long long P = (long long)d->b * ho * wo;
-    long long K = (long long)d->c * d->kh * d->kw;
+    const long long P = (long long)d->b * ho * wo;
+    const long long K = (long long)d->c * d->kh * d->kw;
     long long gm, gn, gk; /* the reference's GEMM orientation (src/convGemm.c) */
     long long out_elems;
     if (d->op == CONVGEMM_FWD) {
@@ -381,88 +488,94 @@ static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, 
     if (!in || !x) return set_error(CONVGEMM_ERR_INVALID_VALUE, "input pointer is NULL");
     if (epi && epi->mean && (!epi->inv_std || !epi->gamma || !epi->beta))
         return set_error(CONVGEMM_ERR_INVALID_VALUE, "batch norm needs inv_std, gamma and beta");
+    if (elem == CG_ELEM_FP8 && !scales)
+        return set_error(CONVGEMM_ERR_INVALID_VALUE, "FP8 operands need their per-tensor scales");
 
-    cg_kparams kp;
-    memset(&kp, 0, sizeof(kp));
-    kp.op = opkind(d);
-    kp.B = d->b; kp.H = d->h; kp.W = d->w; kp.C = d->c; kp.KN = d->kn; kp.KH = d->kh; kp.KW = d->kw;
-    kp.vp = d->vpad; kp.hp = d->hpad; kp.vs = d->vstride; kp.hs = d->hstride; kp.vd = d->vdil; kp.hd = d->hdil;
-    kp.Ho = ho; kp.Wo = wo; kp.taps = d->kh * d->kw;
-    kp.P = P;
     kp.out = (float *)out;
     kp.alpha = alpha;
     kp.beta = beta;
-    kp.scale = 1.0f;
+    kp.scale = elem == CG_ELEM_FP8 ? scales[0] * scales[1] : 1.0f;
     if (d->op == CONVGEMM_FWD && epi) {
         kp.bias = epi->bias; kp.mean = epi->mean; kp.inv_std = epi->inv_std;
         kp.gamma = epi->gamma; kp.bnb = epi->beta; kp.relu = epi->relu != 0;
     }
-    /* operands */
-    const float *wsrc = NULL;
-    if (d->op == CONVGEMM_FWD) { kp.A = x; wsrc = (const float *)in; }
-    else if (d->op == CONVGEMM_DGRAD) { kp.A = x; wsrc = (const float *)in; }
-    else { kp.A = x; kp.Bm = in; }
 
-    int need_wt = (d->op != CONVGEMM_WGRAD) && d->layout == CONVGEMM_NCHW;
-    size_t wt_bytes = need_wt ? (size_t)K * d->kn * sizeof(float) : 0;
-    int vec_ok_a = ((uintptr_t)kp.A % 16) == 0;
-    int vec_ok_b = ((uintptr_t)(need_wt ? ws->ws_w : (const void *)wsrc) % 16) == 0 || d->op == CONVGEMM_WGRAD;
-    if (kp.op == CG_FWD_NHWC) kp.vecA = vec_ok_a && d->c % 4 == 0;
-    if (kp.op == CG_DGRAD_NHWC) kp.vecA = vec_ok_a && d->kn % 4 == 0;
-    if (kp.op == CG_DGRAD_NHWC || kp.op == CG_DGRAD_NCHW) kp.vecB = vec_ok_b && d->kn % 4 == 0;
+    /* weights in the order the kernels read */
+    int rd0 = 0, rd1 = 0, rd2 = 0, rperm = 0;
+    const int repack = weight_repack(d, elem, &rd0, &rd1, &rd2, &rperm);
+    const size_t wt_bytes = repack ? (size_t)K * d->kn * es : 0;
+    if (repack && ws->ws_w && ws->w_bytes < wt_bytes)
+        return set_error(CONVGEMM_ERR_WORKSPACE, "weight workspace %zu < %zu bytes", ws->w_bytes, wt_bytes);
 
-    /* plan */
-    int cfg = -1, splits = 1;
-    if (g_force_cfg >= 0) {
-        cfg = g_force_cfg;
-        splits = g_force_splits > 0 ? g_force_splits : 1;
-    } else if (g_plan_hook) {
-        g_plan_hook(g_plan_hook_ctx, &kp, &cfg, &splits);
-    }
-    if (cfg < 0 || !cg_tilecfg_supports(CG_ELEM_F32, cfg, kp.op)) default_plan(&kp, &cfg, &splits);
+    /* operand roles: A = X (FWD, WGRAD) or dY (DGRAD); B = weights (FWD,
+     * DGRAD) or dY (WGRAD) */
+    const void *bsrc = in;
+    kp.A = x;
+    long long img_a, img_o, img_b;
+    image_bytes(d, ho, wo, es, &img_a, &img_o, &img_b);
+
+    /* plan on the first chunk's shape */
+    int cfg, splits;
+    choose_plan(elem, &kp, &cfg, &splits);
     cg_tilecfg t;
-    if (cg_get_tilecfg(CG_ELEM_F32, cfg, &t))
-        return set_error(CONVGEMM_ERR_INVALID_VALUE, "unknown tile configuration %d", cfg);
+    if (cfg < 0 || cg_get_tilecfg(elem, cfg, &t))
+        return set_error(CONVGEMM_ERR_UNSUPPORTED, "no kernel configuration for this problem");
     cg_fill_tiles(&kp, &t, splits);
 
-    /* workspace */
-    if (need_wt) {
-        if (ws->ws_w && ws->w_bytes < wt_bytes)
-            return set_error(CONVGEMM_ERR_WORKSPACE, "weight workspace %zu < %zu bytes", ws->w_bytes, wt_bytes);
-    }
+    /* workspace: repacked weights + split-K partials */
     size_t pbytes = cg_partial_bytes(&kp);
     while (pbytes && ws->ws_parts && pbytes > ws->parts_bytes && kp.splits > 1) {
         cg_fill_tiles(&kp, &t, kp.splits / 2);
         pbytes = cg_partial_bytes(&kp);
     }
-    if ((need_wt && !ws->ws_w) || (pbytes && !ws->ws_parts)) {
+    if ((repack && !ws->ws_w) || (pbytes && !ws->ws_parts)) {
         size_t wt_al = (wt_bytes + 255) & ~(size_t)255;
         void *base = NULL;
         st = internal_ws(wt_al + pbytes, &base);
         if (st != CONVGEMM_SUCCESS) return st;
-        if (need_wt && !ws->ws_w) { ws->ws_w = base; ws->w_bytes = wt_bytes; }
+        if (repack && !ws->ws_w) { ws->ws_w = base; ws->w_bytes = wt_bytes; }
         if (pbytes && !ws->ws_parts) { ws->ws_parts = (char *)base + wt_al; ws->parts_bytes = pbytes; }
     }
-    if (need_wt) {
-        /* NCHW weights [kn][K] -> [K][kn], the NHWC weight order the kernels read */
-        int e = cg_launch_transpose_f32(wsrc, (float *)ws->ws_w, d->kn, (int)K, stream);
-        if (e) return hip_fail((hipError_t)e, "weight transpose");
-        kp.Bm = ws->ws_w;
-        kp.vecB = kp.vecB && (((uintptr_t)ws->ws_w % 16) == 0);
-    } else if (d->op != CONVGEMM_WGRAD) {
-        kp.Bm = wsrc;
+    if (repack) {
+        int e = cg_launch_permute3(in, ws->ws_w, rd0, rd1, rd2, rperm, es, stream);
+        if (e) return hip_fail((hipError_t)e, "weight repack");
+        bsrc = ws->ws_w;
+        if ((uintptr_t)bsrc % 16) kp.vec = 0;
+        if (!cg_tilecfg_supports(elem, cfg, kp.op, kp.vec))
+            return set_error(CONVGEMM_ERR_UNSUPPORTED, "misaligned weight workspace for this configuration");
     }
     kp.ws = pbytes ? (float *)ws->ws_parts : NULL;
 
-    int e = cg_launch_igemm(CG_ELEM_F32, cfg, &kp, stream);
-    if (e) return hip_fail((hipError_t)e, "implicit GEMM launch");
-    if (kp.splits > 1) {
-        e = cg_launch_splitk_reduce(&kp, stream);
-        if (e) return hip_fail((hipError_t)e, "split-K reduction");
+    int nkern = repack;
+    for (int b0 = 0; b0 < d->b; b0 += chunk) {
+        cg_kparams c = kp;
+        int nb = d->b - b0 < chunk ? d->b - b0 : chunk;
+        c.B = nb;
+        c.P = (long long)nb * ho * wo;
+        cg_fill_tiles(&c, &t, kp.splits);
+        c.A = (const char *)x + (size_t)b0 * img_a;
+        c.a_bytes = (long long)nb * img_a;
+        if (d->op == CONVGEMM_WGRAD) {
+            c.Bm = (const char *)in + (size_t)b0 * img_b;
+            c.b_bytes = (long long)nb * img_b;
+            if (b0 > 0) c.beta = 1.0f; /* later chunks accumulate into dW */
+        } else {
+            c.Bm = bsrc;
+            c.b_bytes = (long long)K * d->kn * es;
+            c.out = (float *)((char *)out + (size_t)b0 * img_o);
+        }
+        int e = cg_launch_igemm(elem, cfg, &c, stream);
+        if (e) return hip_fail((hipError_t)e, "implicit GEMM launch");
+        nkern++;
+        if (c.splits > 1) {
+            e = cg_launch_splitk_reduce(&c, stream);
+            if (e) return hip_fail((hipError_t)e, "split-K reduction");
+            nkern++;
+        }
     }
     t_plan_cfg = cfg;
     t_plan_splits = kp.splits;
-    t_plan_kernels = 1 + (kp.splits > 1) + need_wt;
+    t_plan_kernels = nkern;
     return ok();
 }
 
@@ -472,48 +585,106 @@ convgemm_status_t convgemm_conv_ex(const convgemm_desc_t *desc, float alpha, flo
                                    convgemm_stream_t stream) {
     cg_wsets ws = {0};
     if (workspace && ws_bytes) {
-        /* weights first (256-byte aligned), partials after */
+        /* repacked weights first (256-byte aligned), partials after */
         int ho, wo;
         convgemm_status_t st = validate(desc, &ho, &wo);
         if (st != CONVGEMM_SUCCESS) return st;
-        size_t wt = 0;
-        if (desc->op != CONVGEMM_WGRAD && desc->layout == CONVGEMM_NCHW)
-            wt = (size_t)desc->c * desc->kh * desc->kw * desc->kn * sizeof(float);
+        int d0, d1, d2, perm;
+        int elem = elem_of(dtype);
+        size_t wt = weight_repack(desc, elem, &d0, &d1, &d2, &perm)
+                        ? (size_t)desc->c * desc->kh * desc->kw * desc->kn * esize_of(elem) : 0;
         size_t wt_al = (wt + 255) & ~(size_t)255;
         if (wt_al > ws_bytes)
             return set_error(CONVGEMM_ERR_WORKSPACE, "workspace %zu < %zu bytes needed for weights", ws_bytes, wt_al);
         ws.ws_w = wt ? workspace : NULL;
         ws.w_bytes = wt;
+        /* partials after the weights; no room disables split-K */
         ws.ws_parts = (char *)workspace + wt_al;
         ws.parts_bytes = ws_bytes - wt_al;
-        if (!wt) ws.ws_w = workspace; /* unused */
-        if (ws.parts_bytes == 0) ws.ws_parts = NULL;
     }
     return run(desc, alpha, beta, in, x, out, epi, dtype, scales, &ws, (hipStream_t)stream);
+}
+
+convgemm_status_t convgemm_plan(const convgemm_desc_t *d, convgemm_dtype_t dtype, int *cfg, int *splits) {
+    cg_kparams kp;
+    int elem, chunk;
+    if (cfg) *cfg = -1;
+    /* plan as for 256-byte-aligned operands */
+    convgemm_status_t st = cg_prepare(d, dtype, (void *)256, (void *)256, NULL, &kp, &elem, &chunk);
+    if (st != CONVGEMM_SUCCESS) return st;
+    int c, s;
+    choose_plan(elem, &kp, &c, &s);
+    cg_tilecfg t;
+    if (c >= 0 && !cg_get_tilecfg(elem, c, &t)) {
+        cg_fill_tiles(&kp, &t, s);
+        s = kp.splits;
+    }
+    if (cfg) *cfg = c;
+    if (splits) *splits = s;
+    return ok();
 }
 
 size_t convgemm_workspace_size(const convgemm_desc_t *d, convgemm_dtype_t dtype) {
     int ho, wo;
     if (validate(d, &ho, &wo) != CONVGEMM_SUCCESS) return 0;
-    size_t wt = 0;
-    if (d->op != CONVGEMM_WGRAD && d->layout == CONVGEMM_NCHW)
-        wt = (size_t)d->c * d->kh * d->kw * d->kn * sizeof(float);
+    int elem = elem_of(dtype), d0, d1, d2, perm;
+    size_t wt = weight_repack(d, elem, &d0, &d1, &d2, &perm)
+                    ? (size_t)d->c * d->kh * d->kw * d->kn * esize_of(elem) : 0;
     size_t wt_al = (wt + 255) & ~(size_t)255;
-    /* the largest partial buffer the default planner can choose */
+    /* the largest partial buffer any plan may use: 32 splits */
     cg_kparams kp;
     memset(&kp, 0, sizeof(kp));
     kp.op = opkind(d);
     kp.B = d->b; kp.H = d->h; kp.W = d->w; kp.C = d->c; kp.KN = d->kn; kp.KH = d->kh; kp.KW = d->kw;
-    kp.vp = d->vpad; kp.hp = d->hpad; kp.vs = d->vstride; kp.hs = d->hstride; kp.vd = d->vdil; kp.hd = d->hdil;
+    kp.vs = d->vstride; kp.hs = d->hstride;
     kp.Ho = ho; kp.Wo = wo; kp.taps = d->kh * d->kw;
     kp.P = (long long)d->b * ho * wo;
-    int cfg, splits;
-    default_plan(&kp, &cfg, &splits);
-    cg_tilecfg t;
-    cg_get_tilecfg(CG_ELEM_F32, cfg, &t);
-    cg_fill_tiles(&kp, &t, splits);
-    (void)dtype;
-    return wt_al + cg_partial_bytes(&kp) + cg_lowp_extra_bytes(d, dtype);
+    size_t parts = 0;
+    int n = cg_num_tilecfgs(elem);
+    for (int cfg = 0; cfg < n; cfg++) {
+        cg_tilecfg t;
+        if (cg_get_tilecfg(elem, cfg, &t)) continue;
+        cg_fill_tiles(&kp, &t, 32);
+        size_t b = cg_partial_bytes(&kp);
+        if (b > parts) parts = b;
+    }
+    return wt_al + parts;
+}
+
+/* --------------------------------------------------------- quantization */
+
+convgemm_status_t convgemm_quantize(const float *src, void *dst, size_t n, convgemm_dtype_t dtype,
+                                    float *scale_out, convgemm_stream_t stream) {
+    if (dtype != CONVGEMM_BF16 && dtype != CONVGEMM_FP8_E4M3)
+        return set_error(CONVGEMM_ERR_INVALID_VALUE, "quantize target must be BF16 or FP8_E4M3");
+    if (n == 0) {
+        if (scale_out) *scale_out = 1.0f;
+        return ok();
+    }
+    if (!src || !dst) return set_error(CONVGEMM_ERR_INVALID_VALUE, "NULL tensor");
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == CONVGEMM_BF16) {
+        int e = cg_launch_quantize(src, dst, (long long)n, CG_ELEM_BF16, NULL, s);
+        if (e) return hip_fail((hipError_t)e, "bf16 conversion");
+        if (scale_out) *scale_out = 1.0f;
+        return ok();
+    }
+    pthread_mutex_lock(&g_lock);
+    if (!g_qscratch && lib_malloc(&g_qscratch, 256) != hipSuccess) {
+        g_qscratch = NULL;
+        pthread_mutex_unlock(&g_lock);
+        return set_error(CONVGEMM_ERR_ALLOC, "cannot allocate quantizer scratch");
+    }
+    void *scr = g_qscratch;
+    pthread_mutex_unlock(&g_lock);
+    int e = cg_launch_quantize(src, dst, (long long)n, CG_ELEM_FP8, scr, s);
+    if (e) return hip_fail((hipError_t)e, "fp8 quantization");
+    if (scale_out) {
+        hipError_t he = hipMemcpyAsync(scale_out, (char *)scr + 4, sizeof(float), hipMemcpyDeviceToHost, s);
+        if (he == hipSuccess) he = hipStreamSynchronize(s);
+        if (he != hipSuccess) return hip_fail(he, "reading the fp8 scale");
+    }
+    return ok();
 }
 
 /* ------------------------------------------------------------- drop-in */

@@ -1,0 +1,237 @@
+// igemm_aux.hip -- kernels around the implicit GEMM and the launcher entry
+// points of cg_internal.h:
+//   split-K reduction (deterministic, fixed summation order) + epilogue,
+//   beta scaling (the alpha == 0 path of src/quick_gemm.h),
+//   weight repacks (LDS-tiled strided transpose, 4/2/1-byte elements),
+//   FP32 -> BF16 / FP8-e4m3 conversion with per-tensor amax scaling,
+//   per-element-type dispatch of the implicit-GEMM configurations.
+#include "igemm_common.hpp"
+
+namespace cg {
+
+// ---- split-K reduction: one thread per (class, m, n) ----
+template <int OP>
+__global__ void __launch_bounds__(256) splitk_reduce_kernel(const cg_kparams p) {
+    const int cls = blockIdx.y;
+    const Ctx x = make_ctx(p, cls);
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)x.Mrows * p.N;
+    if (idx >= total) return;
+    const int m = (int)(idx / p.N), n = (int)(idx - (long long)m * p.N);
+    const int ncls = p.ncls_h * p.ncls_w;
+    float s = 0.f;
+    for (int z = 0; z < p.splits; z++) s += p.ws[(((long long)z * ncls + cls) * p.M + m) * p.N + n];
+    store_out<OP>(p, x, m, n, s);
+}
+
+__global__ void scale_kernel(float *out, long long n, float beta) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) out[i] = beta == 0.0f ? 0.0f : beta * out[i];
+}
+
+// dst[z*tz + c*tc + r] = src[z*sz + r*sr + c] for r < R, c < Cc, z < gridDim.z:
+// a batch of 2-D transposes through a 32x33 LDS tile.
+template <typename E>
+__global__ void strided_transpose_kernel(const E *__restrict__ src, E *__restrict__ dst, int R, int Cc,
+                                         long long sz, long long sr, long long tz, long long tc) {
+    __shared__ E t[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const long long z = blockIdx.z;
+    for (int i = threadIdx.y; i < 32; i += blockDim.y) {
+        const int r = r0 + i, c = c0 + threadIdx.x;
+        if (r < R && c < Cc) t[i][threadIdx.x] = src[z * sz + (long long)r * sr + c];
+    }
+    __syncthreads();
+    for (int i = threadIdx.y; i < 32; i += blockDim.y) {
+        const int c = c0 + i, r = r0 + threadIdx.x;
+        if (r < R && c < Cc) dst[z * tz + (long long)c * tc + r] = t[threadIdx.x][i];
+    }
+}
+
+// ---- FP32 -> BF16 (round to nearest even) ----
+__device__ inline uint16_t f32_to_bf16_rne(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x0040u);  // quiet NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__global__ void to_bf16_kernel(const float *__restrict__ src, uint16_t *__restrict__ dst, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = f32_to_bf16_rne(src[i]);
+}
+
+// ---- per-tensor amax, then FP32 -> FP8 e4m3fn ----
+__global__ void zero_u32_kernel(unsigned *p) { *p = 0u; }
+
+__global__ void amax_kernel(const float *__restrict__ src, long long n, unsigned *amax_bits) {
+    float m = 0.f;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) m = fmaxf(m, fabsf(src[i]));
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    __shared__ float wmax[16];
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmaxf(m, wmax[w]);
+        // non-negative floats order like their bit patterns
+        atomicMax(amax_bits, __builtin_bit_cast(unsigned, m));
+    }
+}
+
+// stored = e4m3fn(clamp(x / scale, -448, 448)), scale = amax / 448 (1 if amax == 0 or not finite)
+__global__ void to_fp8_kernel(const float *__restrict__ src, uint8_t *__restrict__ dst, long long n,
+                              const unsigned *amax_bits, float *scale_out) {
+    const float amax = __builtin_bit_cast(float, *amax_bits);
+    const float scale = (amax > 0.f && amax < 3.0e38f) ? amax / 448.0f : 1.0f;
+    const float inv = 1.0f / scale;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *scale_out = scale;
+    long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * 2;
+    const long long stride = (long long)gridDim.x * blockDim.x * 2;
+    for (; i < n; i += stride) {
+        const float a = fminf(fmaxf(src[i] * inv, -448.0f), 448.0f);
+        const float b = (i + 1 < n) ? fminf(fmaxf(src[i + 1] * inv, -448.0f), 448.0f) : 0.0f;
+        const int packed = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+        dst[i] = (uint8_t)(packed & 0xff);
+        if (i + 1 < n) dst[i + 1] = (uint8_t)((packed >> 8) & 0xff);
+    }
+}
+
+template <int OP>
+static void launch_reduce(const cg_kparams *kp, dim3 grid, hipStream_t st) {
+    hipLaunchKernelGGL(splitk_reduce_kernel<OP>, grid, dim3(256), 0, st, *kp);
+}
+
+}  // namespace cg
+
+using namespace cg;
+
+// ---- per-type configuration tables (igemm_{f32,bf16,fp8}.hip) ----
+extern "C" {
+int cg_f32_num_cfgs(void);
+int cg_f32_get_cfg(int cfg, cg_tilecfg *out);
+int cg_f32_supports(int cfg, int op, int vec);
+int cg_f32_launch(int cfg, const cg_kparams *kp, void *stream);
+int cg_bf16_num_cfgs(void);
+int cg_bf16_get_cfg(int cfg, cg_tilecfg *out);
+int cg_bf16_supports(int cfg, int op, int vec);
+int cg_bf16_launch(int cfg, const cg_kparams *kp, void *stream);
+int cg_fp8_num_cfgs(void);
+int cg_fp8_get_cfg(int cfg, cg_tilecfg *out);
+int cg_fp8_supports(int cfg, int op, int vec);
+int cg_fp8_launch(int cfg, const cg_kparams *kp, void *stream);
+}
+
+extern "C" int cg_num_tilecfgs(int elem) {
+    switch (elem) {
+    case CG_ELEM_F32: return cg_f32_num_cfgs();
+    case CG_ELEM_BF16: return cg_bf16_num_cfgs();
+    case CG_ELEM_FP8: return cg_fp8_num_cfgs();
+    }
+    return 0;
+}
+
+extern "C" int cg_get_tilecfg(int elem, int cfg, cg_tilecfg *out) {
+    switch (elem) {
+    case CG_ELEM_F32: return cg_f32_get_cfg(cfg, out);
+    case CG_ELEM_BF16: return cg_bf16_get_cfg(cfg, out);
+    case CG_ELEM_FP8: return cg_fp8_get_cfg(cfg, out);
+    }
+    return -1;
+}
+
+extern "C" int cg_tilecfg_supports(int elem, int cfg, int op, int vec) {
+    switch (elem) {
+    case CG_ELEM_F32: return cg_f32_supports(cfg, op, vec);
+    case CG_ELEM_BF16: return cg_bf16_supports(cfg, op, vec);
+    case CG_ELEM_FP8: return cg_fp8_supports(cfg, op, vec);
+    }
+    return 0;
+}
+
+extern "C" int cg_launch_igemm(int elem, int cfg, const cg_kparams *kp, void *stream) {
+    switch (elem) {
+    case CG_ELEM_F32: return cg_f32_launch(cfg, kp, stream);
+    case CG_ELEM_BF16: return cg_bf16_launch(cfg, kp, stream);
+    case CG_ELEM_FP8: return cg_fp8_launch(cfg, kp, stream);
+    }
+    return (int)hipErrorInvalidValue;
+}
+
+extern "C" int cg_launch_splitk_reduce(const cg_kparams *kp, void *stream) {
+    hipStream_t st = (hipStream_t)stream;
+    const long long total = (long long)kp->M * kp->N;  // rows of the largest class bound every class
+    dim3 grid((unsigned)((total + 255) / 256), kp->ncls_h * kp->ncls_w);
+    switch (kp->op) {
+    case CG_FWD_NHWC: launch_reduce<CG_FWD_NHWC>(kp, grid, st); break;
+    case CG_FWD_NCHW: launch_reduce<CG_FWD_NCHW>(kp, grid, st); break;
+    case CG_WGRAD_NHWC: launch_reduce<CG_WGRAD_NHWC>(kp, grid, st); break;
+    case CG_WGRAD_NCHW: launch_reduce<CG_WGRAD_NCHW>(kp, grid, st); break;
+    case CG_DGRAD_NHWC: launch_reduce<CG_DGRAD_NHWC>(kp, grid, st); break;
+    case CG_DGRAD_NCHW: launch_reduce<CG_DGRAD_NCHW>(kp, grid, st); break;
+    default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
+
+extern "C" int cg_launch_scale(float *out, long long n, float beta, void *stream) {
+    if (n <= 0) return 0;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(scale_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, out, n, beta);
+    return (int)hipGetLastError();
+}
+
+template <typename E>
+static int launch_transpose(const void *src, void *dst, int R, int Cc, int batch, long long sz, long long sr,
+                            long long tz, long long tc, hipStream_t st) {
+    dim3 grid((Cc + 31) / 32, (R + 31) / 32, batch);
+    hipLaunchKernelGGL(strided_transpose_kernel<E>, grid, dim3(32, 8), 0, st, (const E *)src, (E *)dst, R, Cc, sz,
+                       sr, tz, tc);
+    return (int)hipGetLastError();
+}
+
+extern "C" int cg_launch_permute3(const void *src, void *dst, int d0, int d1, int d2, int perm, int esize,
+                                  void *stream) {
+    if (d0 <= 0 || d1 <= 0 || d2 <= 0) return 0;
+    int R, Cc, batch;
+    long long sz, sr, tz, tc;
+    if (perm == 0) {  // [d0][d1][d2] -> [d2][d1][d0]: batch over d1
+        R = d0; Cc = d2; batch = d1;
+        sz = d2; sr = (long long)d1 * d2; tz = d0; tc = (long long)d1 * d0;
+    } else {          // [d0][d1][d2] -> [d0][d2][d1]: batch over d0
+        R = d1; Cc = d2; batch = d0;
+        sz = (long long)d1 * d2; sr = d2; tz = (long long)d2 * d1; tc = d1;
+    }
+    if (batch > 65535) return (int)hipErrorInvalidValue;
+    hipStream_t st = (hipStream_t)stream;
+    switch (esize) {
+    case 4: return launch_transpose<float>(src, dst, R, Cc, batch, sz, sr, tz, tc, st);
+    case 2: return launch_transpose<uint16_t>(src, dst, R, Cc, batch, sz, sr, tz, tc, st);
+    case 1: return launch_transpose<uint8_t>(src, dst, R, Cc, batch, sz, sr, tz, tc, st);
+    }
+    return (int)hipErrorInvalidValue;
+}
+
+// FP32 -> BF16 (scale = 1) or FP8 e4m3fn with a per-tensor amax scale.
+// `scratch` holds 8 device bytes (amax bits + scale); `scale_dev` receives the scale.
+extern "C" int cg_launch_quantize(const float *src, void *dst, long long n, int elem, void *scratch, void *stream) {
+    hipStream_t st = (hipStream_t)stream;
+    if (n <= 0) return 0;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (elem == CG_ELEM_BF16) {
+        hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)blocks), dim3(256), 0, st, src, (uint16_t *)dst, n);
+        return (int)hipGetLastError();
+    }
+    if (elem != CG_ELEM_FP8) return (int)hipErrorInvalidValue;
+    unsigned *amax = (unsigned *)scratch;
+    float *scale = (float *)scratch + 1;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, st, amax);
+    hipLaunchKernelGGL(amax_kernel, dim3((unsigned)blocks), dim3(256), 0, st, src, n, amax);
+    hipLaunchKernelGGL(to_fp8_kernel, dim3((unsigned)blocks), dim3(256), 0, st, src, (uint8_t *)dst, n, amax, scale);
+    return (int)hipGetLastError();
+}

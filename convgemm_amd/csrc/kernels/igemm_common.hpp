@@ -11,6 +11,55 @@ namespace cg {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned char u8x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+
+// Operand element types: float (FP32), uint16_t (bfloat16 bits), uint8_t
+// (OCP fp8 e4m3fn bits).  VEC elements make one 16-byte chunk; G is the
+// reduction length one MFMA group covers per lane pair (f32: 8 x 32x32x2,
+// bf16: 32x32x16, fp8: 32x32x64).
+template <typename T> struct ET;
+template <> struct ET<float> {
+    static constexpr int VEC = 4, G = 16;
+    typedef f32x4 V;
+};
+template <> struct ET<uint16_t> {
+    static constexpr int VEC = 8, G = 16;
+    typedef u16x8 V;
+};
+template <> struct ET<uint8_t> {
+    static constexpr int VEC = 16, G = 64;
+    typedef u8x16 V;
+};
+
+// Buffer-resource loads: an offset at or beyond the resource size returns 0
+// in hardware, so masked gathers need neither branches nor selects.
+// Operand tensors are limited to < 2 GiB per launch (the host splits larger
+// problems along the batch); OOB is an offset no resource reaches.
+constexpr unsigned OOB = 0x80000000u;
+
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void *p, long long bytes) {
+    unsigned n = bytes >= (long long)OOB ? OOB : (unsigned)bytes;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)n, 0x00020000);
+}
+
+template <typename T>
+__device__ inline typename ET<T>::V ld_vec(__amdgpu_buffer_rsrc_t r, unsigned boff) {
+    return __builtin_bit_cast(typename ET<T>::V, __builtin_amdgcn_raw_buffer_load_b128(r, boff, 0, 0));
+}
+template <typename T> __device__ inline T ld_one(__amdgpu_buffer_rsrc_t r, unsigned boff);
+template <> __device__ inline float ld_one<float>(__amdgpu_buffer_rsrc_t r, unsigned boff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, boff, 0, 0));
+}
+template <> __device__ inline uint16_t ld_one<uint16_t>(__amdgpu_buffer_rsrc_t r, unsigned boff) {
+    return __builtin_amdgcn_raw_buffer_load_b16(r, boff, 0, 0);
+}
+template <> __device__ inline uint8_t ld_one<uint8_t>(__amdgpu_buffer_rsrc_t r, unsigned boff) {
+    return __builtin_amdgcn_raw_buffer_load_b8(r, boff, 0, 0);
+}
 
 // Division by a run-time constant d >= 1 for 0 <= n < 2^31
 // (Granlund-Montgomery round-up method): q = (n * m) >> s with
@@ -104,7 +153,7 @@ __device__ inline void store_out(const cg_kparams &p, const Ctx &x, int m, int n
         else
             addr = (long long)n * ((long long)p.C * p.taps) + (long long)c * p.taps + tap;
     }
-    float t = p.alpha * acc;
+    float t = p.alpha * p.scale * acc;
     if (p.beta != 0.0f) t += p.beta * p.out[addr];
     if constexpr (OP == CG_FWD_NHWC || OP == CG_FWD_NCHW) {
         if (p.bias) t += p.bias[n];

@@ -1,0 +1,48 @@
+// igemm_fp8.hip -- FP8 e4m3 (v_mfma_scale_f32_32x32x64_f8f6f4) instantiations of the implicit-GEMM kernel and
+// its tile-configuration table (BM, BN, BK, waves along M, waves along N).
+#include "igemm_kernel.hpp"
+
+using namespace cg;
+
+namespace {
+
+const cg_tilecfg kCfg[] = {
+    {128, 128, 64, 2, 2},  // 0
+    {128, 64, 64, 2, 2},  // 1
+    {64, 128, 64, 2, 2},  // 2
+    {64, 64, 64, 2, 2},  // 3
+    {256, 64, 64, 4, 1},  // 4
+    {256, 128, 64, 2, 2},  // 5
+    {128, 128, 128, 2, 2},  // 6
+    {128, 256, 64, 2, 2},  // 7
+};
+constexpr int kNum = sizeof(kCfg) / sizeof(kCfg[0]);
+// configurations that also have element-load (non-16-byte) instantiations
+const bool kNoVec[kNum] = {true, false, false, true, false, false, false, false};
+
+}  // namespace
+
+extern "C" int cg_fp8_num_cfgs(void) { return kNum; }
+
+extern "C" int cg_fp8_get_cfg(int cfg, cg_tilecfg *out) {
+    if (cfg < 0 || cfg >= kNum) return -1;
+    *out = kCfg[cfg];
+    return 0;
+}
+
+extern "C" int cg_fp8_supports(int cfg, int op, int vec) { return supports<uint8_t>(kCfg, kNoVec, kNum, cfg, op, vec); }
+
+extern "C" int cg_fp8_launch(int cfg, const cg_kparams *kp, void *stream) {
+    hipStream_t st = (hipStream_t)stream;
+    switch (cfg) {
+    case 0: return launch_cfg<uint8_t, 128, 128, 64, 2, 2, true>(kp, st);
+    case 1: return launch_cfg<uint8_t, 128, 64, 64, 2, 2, false>(kp, st);
+    case 2: return launch_cfg<uint8_t, 64, 128, 64, 2, 2, false>(kp, st);
+    case 3: return launch_cfg<uint8_t, 64, 64, 64, 2, 2, true>(kp, st);
+    case 4: return launch_cfg<uint8_t, 256, 64, 64, 4, 1, false>(kp, st);
+    case 5: return launch_cfg<uint8_t, 256, 128, 64, 2, 2, false>(kp, st);
+    case 6: return launch_cfg<uint8_t, 128, 128, 128, 2, 2, false>(kp, st);
+    case 7: return launch_cfg<uint8_t, 128, 256, 64, 2, 2, false>(kp, st);
+    }
+    return (int)hipErrorInvalidValue;
+}

@@ -1,0 +1,570 @@
+// igemm_kernel.hpp -- the implicit-GEMM convolution kernel template for gfx950.
+//
+// One template serves the six operations of convGemm (src/convGemm.c:48-146)
+// and three operand types (FP32, BF16, FP8-e4m3; FP32 accumulation).  The
+// reference packs im2row/im2col panels on the CPU (src/im2row_nhwc.c:28-145,
+// src/im2col_nchw.c:29-144, :238-335) and runs a BLIS micro-kernel on them;
+// here each workgroup gathers its convolution patches straight from HBM
+// (buffer loads: out-of-range taps and padding read as 0 in hardware) into an
+// LDS double buffer and contracts them on MFMA.  The reference's post-process
+// callbacks (bias/BN/ReLU epilogue, NCHW transposed store, row2im/col2im
+// scatter) become the write-back of the accumulators: backward data is a
+// gather GEMM per stride residue class (one writer per dX element, no
+// atomics, no multiply by a structural zero).
+//
+// Tile: BM x BN outputs per 256-thread workgroup, BK reduction elements per
+// K step, 4 waves arranged WM x WN, each wave owning (BM/WM) x (BN/WN) as
+// 32x32 MFMA blocks.  LDS rows are K-contiguous, padded by one 16-byte chunk.
+#pragma once
+#include <type_traits>
+
+#include "igemm_common.hpp"
+
+namespace cg {
+
+constexpr int NT = 256;
+
+enum LoadKind { LK_KV = 0, LK_RS = 1, LK_KS = 2 };
+
+template <int OP> constexpr bool is_fwd() { return OP == CG_FWD_NHWC || OP == CG_FWD_NCHW; }
+template <int OP> constexpr bool is_dgrad() { return OP == CG_DGRAD_NHWC || OP == CG_DGRAD_NCHW; }
+template <int OP> constexpr bool is_wgrad() { return OP == CG_WGRAD_NHWC || OP == CG_WGRAD_NCHW; }
+// NCHW outputs are contiguous along M: put M on the MFMA columns (lanes).
+template <int OP> constexpr bool swap_out() {
+    return OP == CG_FWD_NCHW || OP == CG_DGRAD_NCHW || OP == CG_WGRAD_NCHW;
+}
+// Forward weights: FP32 reads the caller's [c][kh][kw][kn] order (rows = kn
+// walk contiguous memory); BF16/FP8 read a [kn][kh][kw][c] repack so that a
+// 16-byte chunk of 8/16 channels is one vector load.
+template <typename T> constexpr bool fwd_b_kmajor() { return !std::is_same<T, float>::value; }
+
+// How each operand tile is read from HBM:
+//   KV: contiguous along K, one 16-byte chunk per load (VEC), else element loads
+//   KS: contiguous along K, element loads (unaligned runs)
+//   RS: contiguous along the row dimension (lanes walk rows), element loads
+template <int OP> constexpr int a_kind() {
+    return OP == CG_FWD_NHWC ? LK_KV : OP == CG_FWD_NCHW ? LK_RS : OP == CG_DGRAD_NHWC ? LK_KV
+           : OP == CG_DGRAD_NCHW ? LK_RS : OP == CG_WGRAD_NHWC ? LK_RS : LK_KS;
+}
+template <typename T, int OP> constexpr int b_kind() {
+    return is_fwd<OP>() ? (fwd_b_kmajor<T>() ? LK_KV : LK_RS) : is_dgrad<OP>() ? LK_KV
+           : OP == CG_WGRAD_NHWC ? LK_RS : LK_KS;
+}
+
+template <int KIND, int R, int KQ> constexpr bool loader_ok() {
+    if (R * KQ < NT) return false;
+    if (KIND == LK_RS) return NT % R == 0;
+    return (R * KQ) % NT == 0;
+}
+
+// Thread -> (row, chunk) of a loader.
+template <int KIND, int R, int KQ>
+__device__ inline int ld_row(int tid, int i) {
+    if constexpr (KIND == LK_RS) return tid % R;
+    else return tid / KQ + i * (NT / KQ);
+}
+template <int KIND, int R, int KQ>
+__device__ inline int ld_kq(int tid, int i) {
+    if constexpr (KIND == LK_RS) return tid / R + i * (NT / R);
+    else return tid % KQ;
+}
+
+constexpr int FAR = -(1 << 29);  // row coordinate that fails every bounds test
+
+template <typename T, int BM, int BN, int BK, int WM, int WN, int OP, bool VEC>
+__global__ void __launch_bounds__(NT) igemm_kernel(const cg_kparams p) {
+    typedef typename ET<T>::V V;
+    constexpr int VW = ET<T>::VEC;
+    constexpr int G = ET<T>::G;
+    constexpr int ES = sizeof(T);
+    constexpr int PITCH = BK + VW;
+    constexpr int KQ = BK / VW;
+    constexpr int PA = BM * KQ / NT;
+    constexpr int PB = BN * KQ / NT;
+    constexpr int AK = a_kind<OP>();
+    constexpr int BKIND = b_kind<T, OP>();
+    constexpr bool FW = is_fwd<OP>(), DG = is_dgrad<OP>(), WG = is_wgrad<OP>();
+    constexpr bool SW = swap_out<OP>();
+    constexpr int NRA = (AK == LK_RS) ? 1 : PA;  // distinct A rows per thread
+    constexpr int NRB = (BKIND == LK_RS) ? 1 : PB;
+    static_assert(loader_ok<AK, BM, KQ>() && loader_ok<BKIND, BN, KQ>(), "loader geometry");
+    static_assert(BK % G == 0, "BK must be a multiple of the MFMA group");
+    constexpr int WTM = BM / WM, WTN = BN / WN;
+    constexpr int ER = SW ? WTN : WTM, EC = SW ? WTM : WTN;
+    constexpr int TR = ER / 32, TC = EC / 32;
+    static_assert(TR >= 1 && TC >= 1 && ER % 32 == 0 && EC % 32 == 0, "wave tile");
+
+    __shared__ __attribute__((aligned(16))) T sA[2 * BM * PITCH];
+    __shared__ __attribute__((aligned(16))) T sB[2 * BN * PITCH];
+    __shared__ int s_kx[64], s_ky[64], s_nk[2];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+    // ---- block -> (tile, split, class).  Workgroups are dealt to the 8 XCDs
+    // round-robin; give each XCD a contiguous run of tiles so neighbouring
+    // tiles (shared weight panel, shared input halo) meet in one L2.
+    const int nb = gridDim.x, bid = blockIdx.x;
+    const int xcd = bid & 7, per = nb >> 3, rem = nb & 7;
+    const int lin = xcd * per + min(xcd, rem) + (bid >> 3);
+    const int tiles = p.mtiles * p.ntiles;
+    const int tile = lin % tiles, rest = lin / tiles;
+    const int split = rest % p.splits, cls = rest / p.splits;
+    const int mt = tile % p.mtiles, nt = tile / p.mtiles;
+
+    const Ctx x = make_ctx(p, cls);
+    if (!WG && mt * BM >= x.Mrows) return;  // uniform: DGRAD classes smaller than the grid
+
+    const __amdgpu_buffer_rsrc_t rA = make_rsrc(p.A, p.a_bytes);
+    const __amdgpu_buffer_rsrc_t rB = make_rsrc(p.Bm, p.b_bytes);
+    const int HoWo = p.Ho * p.Wo;
+
+    // ---- reduction range of this block ----
+    int nky = p.KW, nkx = p.KH;
+    if constexpr (DG) {
+        if (tid == 0) {
+            int n = 0;
+            for (int kx = 0; kx < p.KH; kx++)
+                if ((x.rh + p.vp - kx * p.vd) % p.vs == 0) s_kx[n++] = kx;
+            s_nk[0] = n;
+            n = 0;
+            for (int ky = 0; ky < p.KW; ky++)
+                if ((x.rw + p.hp - ky * p.hd) % p.hs == 0) s_ky[n++] = ky;
+            s_nk[1] = n;
+        }
+        __syncthreads();
+        nkx = s_nk[0];
+        nky = s_nk[1];
+    }
+    const int KT = WG ? p.KT : nkx * nky * p.kchunks;
+    const int per_split = (KT + p.splits - 1) / p.splits;
+    const int kt_begin = min(KT, split * per_split);
+    const int kt_end = min(KT, kt_begin + per_split);
+
+    // WGRAD: this M tile is (tap, c0 .. c0+BM)
+    int wg_tap = 0, wg_c0 = 0, wg_kx = 0, wg_ky = 0;
+    if constexpr (WG) {
+        wg_tap = mt / p.kchunks;
+        wg_c0 = (mt - wg_tap * p.kchunks) * BM;
+        wg_kx = wg_tap / p.KW;
+        wg_ky = wg_tap - wg_kx * p.KW;
+    }
+
+    // ---- per-thread row state ----
+    int a_b[NRA], a_h[NRA], a_w[NRA];  // FWD: b, x*vs-vp, y*hs-hp; DGRAD: b, i', j'; WGRAD: -, c, -
+#pragma unroll
+    for (int i = 0; i < NRA; i++) {
+        const int row = ld_row<AK, BM, KQ>(tid, i);
+        a_b[i] = 0;
+        a_h[i] = FAR;
+        a_w[i] = FAR;
+        if constexpr (FW) {
+            const int m = mt * BM + row;
+            if (m < (int)p.P) {
+                const int b = fdiv(m, x.howo);
+                const int pix = m - b * HoWo;
+                const int xo = fdiv(pix, x.wo);
+                a_b[i] = b;
+                a_h[i] = xo * p.vs - p.vp;
+                a_w[i] = (pix - xo * p.Wo) * p.hs - p.hp;
+            }
+        } else if constexpr (DG) {
+            const int q = mt * BM + row;
+            if (q < x.Mrows) {
+                const int b = fdiv(q, x.cls_hw);
+                const int rem2 = q - b * (x.Hc * x.Wc);
+                const int ii = fdiv(rem2, x.cls_w);
+                a_b[i] = b;
+                a_h[i] = ii;
+                a_w[i] = rem2 - ii * x.Wc;
+            }
+        } else {
+            const int c = wg_c0 + row;
+            a_h[i] = c < p.C ? c : -1;
+        }
+    }
+    int b_row[NRB];  // FWD/WGRAD: kn index (or -1); DGRAD: c index (or -1)
+#pragma unroll
+    for (int i = 0; i < NRB; i++) {
+        const int r = nt * BN + ld_row<BKIND, BN, KQ>(tid, i);
+        b_row[i] = r < (DG ? p.C : p.KN) ? r : -1;
+    }
+
+    V ra[PA], rb[PB];
+
+    // reduction iterator (FWD/DGRAD): tile kt -> (tap index ti, channel chunk ch)
+    int it_ch = 0, it_ti = 0, it_kx = 0, it_ky = 0, it_vx = 0, it_vy = 0;
+    auto it_set = [&](int kt) {
+        it_ti = kt / p.kchunks;
+        it_ch = kt - it_ti * p.kchunks;
+        if constexpr (DG) {
+            if (nky > 0) {
+                it_vx = it_ti / nky;
+                it_vy = it_ti - it_vx * nky;
+                it_kx = s_kx[it_vx];
+                it_ky = s_ky[it_vy];
+            }
+        } else {
+            it_kx = it_ti / p.KW;
+            it_ky = it_ti - it_kx * p.KW;
+        }
+    };
+    auto it_next = [&]() {
+        if (++it_ch == p.kchunks) {
+            it_ch = 0;
+            ++it_ti;
+            if constexpr (DG) {
+                if (++it_vy == nky) { it_vy = 0; ++it_vx; }
+                it_kx = s_kx[min(it_vx, 63)];
+                it_ky = s_ky[it_vy];
+            } else {
+                if (++it_ky == p.KW) { it_ky = 0; ++it_kx; }
+            }
+        }
+    };
+
+    // ---- gather one K tile (global -> registers) ----
+    auto load_tile = [&](int kt) {
+        if constexpr (!WG) {
+            const int r0 = it_ch * BK;
+            const int kx = it_kx, ky = it_ky;
+            const int tap = kx * p.KW + ky;
+            const int cred = FW ? p.C : p.KN;  // reduction channels
+            int dh, dw, lim_h, lim_w;
+            if constexpr (FW) {
+                dh = kx * p.vd; dw = ky * p.hd; lim_h = p.H; lim_w = p.W;
+            } else {
+                dh = (x.rh + p.vp - kx * p.vd) / p.vs;
+                dw = (x.rw + p.hp - ky * p.hd) / p.hs;
+                lim_h = p.Ho; lim_w = p.Wo;
+            }
+            // A operand
+            if constexpr (AK == LK_KV) {  // NHWC: chunk = VW consecutive channels of one pixel
+                const int c = r0 + ld_kq<AK, BM, KQ>(tid, 0) * VW;
+#pragma unroll
+                for (int i = 0; i < PA; i++) {
+                    const int ih = a_h[i] + dh, iw = a_w[i] + dw;
+                    const bool okp = (unsigned)ih < (unsigned)lim_h && (unsigned)iw < (unsigned)lim_w;
+                    const unsigned base =
+                        (((unsigned)a_b[i] * lim_h + (unsigned)ih) * lim_w + (unsigned)iw) * cred + c;
+                    if constexpr (VEC) {
+                        ra[i] = ld_vec<T>(rA, (okp && c < cred) ? base * ES : OOB);
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < VW; j++)
+                            ra[i][j] = ld_one<T>(rA, (okp && c + j < cred) ? (base + j) * ES : OOB);
+                    }
+                }
+            } else {  // NCHW: lanes walk pixels, one plane per channel
+                const int ih = a_h[0] + dh, iw = a_w[0] + dw;
+                const bool okp = (unsigned)ih < (unsigned)lim_h && (unsigned)iw < (unsigned)lim_w;
+                const unsigned plane = (unsigned)lim_h * lim_w;
+                const unsigned base = (unsigned)a_b[0] * cred * plane + (unsigned)ih * lim_w + (unsigned)iw;
+#pragma unroll
+                for (int i = 0; i < PA; i++) {
+                    const int c = r0 + ld_kq<AK, BM, KQ>(tid, i) * VW;
+#pragma unroll
+                    for (int j = 0; j < VW; j++)
+                        ra[i][j] = ld_one<T>(rA, (okp && c + j < cred) ? (base + (unsigned)(c + j) * plane) * ES : OOB);
+                }
+            }
+            // B operand
+            if constexpr (FW && !fwd_b_kmajor<T>()) {  // [c][tap][kn]: lanes walk kn
+                const int n = b_row[0];
+#pragma unroll
+                for (int i = 0; i < PB; i++) {
+                    const int c = r0 + ld_kq<BKIND, BN, KQ>(tid, i) * VW;
+#pragma unroll
+                    for (int j = 0; j < VW; j++)
+                        rb[i][j] = ld_one<T>(rB, (n >= 0 && c + j < p.C)
+                                                     ? (((unsigned)(c + j) * p.taps + tap) * p.KN + n) * ES : OOB);
+                }
+            } else {  // FWD LP: [kn][tap][c];  DGRAD: [c][tap][kn] -- contiguous along the reduction
+                const int k0 = r0 + ld_kq<BKIND, BN, KQ>(tid, 0) * VW;
+#pragma unroll
+                for (int i = 0; i < PB; i++) {
+                    const int row = b_row[i];
+                    const unsigned base = ((unsigned)row * p.taps + tap) * cred + k0;
+                    if constexpr (VEC) {
+                        rb[i] = ld_vec<T>(rB, (row >= 0 && k0 < cred) ? base * ES : OOB);
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < VW; j++)
+                            rb[i][j] = ld_one<T>(rB, (row >= 0 && k0 + j < cred) ? (base + j) * ES : OOB);
+                    }
+                }
+            }
+        } else {
+            // WGRAD: K = output pixels p0 .. p0+BK
+            const int p0 = kt * BK;
+            const int P = (int)p.P;
+            if constexpr (OP == CG_WGRAD_NHWC) {
+                const int c = a_h[0];
+#pragma unroll
+                for (int i = 0; i < PA; i++) {
+                    const int pp = p0 + ld_kq<AK, BM, KQ>(tid, i) * VW;
+                    int b = 0, xo = 0, yo = 0;
+                    if (pp < P) {
+                        b = fdiv(pp, x.howo);
+                        const int pix = pp - b * HoWo;
+                        xo = fdiv(pix, x.wo);
+                        yo = pix - xo * p.Wo;
+                    }
+#pragma unroll
+                    for (int j = 0; j < VW; j++) {
+                        const int ih = xo * p.vs - p.vp + wg_kx * p.vd;
+                        const int iw = yo * p.hs - p.hp + wg_ky * p.hd;
+                        const bool ok = c >= 0 && pp + j < P && (unsigned)ih < (unsigned)p.H &&
+                                        (unsigned)iw < (unsigned)p.W;
+                        ra[i][j] = ld_one<T>(rA, ok ? ((((unsigned)b * p.H + ih) * p.W + iw) * p.C + c) * ES : OOB);
+                        if (++yo == p.Wo) { yo = 0; if (++xo == p.Ho) { xo = 0; ++b; } }
+                    }
+                }
+                const int n = b_row[0];
+#pragma unroll
+                for (int i = 0; i < PB; i++) {
+                    const int pp = p0 + ld_kq<BKIND, BN, KQ>(tid, i) * VW;
+#pragma unroll
+                    for (int j = 0; j < VW; j++)
+                        rb[i][j] = ld_one<T>(rB, (n >= 0 && pp + j < P) ? ((unsigned)(pp + j) * p.KN + n) * ES : OOB);
+                }
+            } else {  // WGRAD_NCHW: both operands contiguous along pixels
+                const int pp = p0 + ld_kq<AK, BM, KQ>(tid, 0) * VW;
+                int b = 0, xo = 0, yo = 0;
+                if (pp < P) {
+                    b = fdiv(pp, x.howo);
+                    const int pix = pp - b * HoWo;
+                    xo = fdiv(pix, x.wo);
+                    yo = pix - xo * p.Wo;
+                }
+                unsigned xoff[VW], yoff[VW];
+                bool okx[VW], oky[VW];
+#pragma unroll
+                for (int j = 0; j < VW; j++) {
+                    const int ih = xo * p.vs - p.vp + wg_kx * p.vd;
+                    const int iw = yo * p.hs - p.hp + wg_ky * p.hd;
+                    oky[j] = pp + j < P;
+                    okx[j] = oky[j] && (unsigned)ih < (unsigned)p.H && (unsigned)iw < (unsigned)p.W;
+                    xoff[j] = (unsigned)b * p.C * p.H * p.W + (unsigned)ih * p.W + (unsigned)iw;
+                    yoff[j] = (unsigned)b * p.KN * HoWo + (unsigned)(xo * p.Wo + yo);
+                    if (++yo == p.Wo) { yo = 0; if (++xo == p.Ho) { xo = 0; ++b; } }
+                }
+#pragma unroll
+                for (int i = 0; i < PA; i++) {
+                    const int c = a_h[i];
+#pragma unroll
+                    for (int j = 0; j < VW; j++)
+                        ra[i][j] = ld_one<T>(rA, (c >= 0 && okx[j]) ? (xoff[j] + (unsigned)c * p.H * p.W) * ES : OOB);
+                }
+#pragma unroll
+                for (int i = 0; i < PB; i++) {
+                    const int n = b_row[i];
+#pragma unroll
+                    for (int j = 0; j < VW; j++)
+                        rb[i][j] = ld_one<T>(rB, (n >= 0 && oky[j]) ? (yoff[j] + (unsigned)n * HoWo) * ES : OOB);
+                }
+            }
+        }
+    };
+
+    auto store_tile = [&](int buf) {
+        T *dA = sA + buf * BM * PITCH;
+        T *dB = sB + buf * BN * PITCH;
+#pragma unroll
+        for (int i = 0; i < PA; i++)
+            *(V *)(dA + ld_row<AK, BM, KQ>(tid, i) * PITCH + ld_kq<AK, BM, KQ>(tid, i) * VW) = ra[i];
+#pragma unroll
+        for (int i = 0; i < PB; i++)
+            *(V *)(dB + ld_row<BKIND, BN, KQ>(tid, i) * PITCH + ld_kq<BKIND, BN, KQ>(tid, i) * VW) = rb[i];
+    };
+
+    // ---- MFMA main loop ----
+    const int wm = wave % WM, wn = wave / WM;
+    const int roff = SW ? wn * WTN : wm * WTM;  // wave origin along MFMA rows (within tile)
+    const int coff = SW ? wm * WTM : wn * WTN;
+    f32x16 acc[TR][TC];
+#pragma unroll
+    for (int a = 0; a < TR; a++)
+#pragma unroll
+        for (int b = 0; b < TC; b++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc[a][b][v] = 0.f;
+
+    auto compute_tile = [&](int buf) {
+        const T *Lr = SW ? sB + buf * BN * PITCH : sA + buf * BM * PITCH;
+        const T *Lc = SW ? sA + buf * BM * PITCH : sB + buf * BN * PITCH;
+        const int l32 = lane & 31;
+        // Lane half h supplies the reduction elements [g + h*G/2, g + (h+1)*G/2)
+        // of group g; A and B use the same assignment, so any order within the
+        // group gives the same sum.
+        const int hoff = (lane >> 5) * (G / 2);
+#pragma unroll
+        for (int g = 0; g < BK; g += G) {
+            if constexpr (std::is_same<T, float>::value) {
+                f32x4 ar[TR][2], ac[TC][2];
+#pragma unroll
+                for (int a = 0; a < TR; a++) {
+                    const float *s = Lr + (roff + a * 32 + l32) * PITCH + g + hoff;
+                    ar[a][0] = *(const f32x4 *)s;
+                    ar[a][1] = *(const f32x4 *)(s + 4);
+                }
+#pragma unroll
+                for (int b = 0; b < TC; b++) {
+                    const float *s = Lc + (coff + b * 32 + l32) * PITCH + g + hoff;
+                    ac[b][0] = *(const f32x4 *)s;
+                    ac[b][1] = *(const f32x4 *)(s + 4);
+                }
+#pragma unroll
+                for (int s = 0; s < 8; s++)
+#pragma unroll
+                    for (int a = 0; a < TR; a++)
+#pragma unroll
+                        for (int b = 0; b < TC; b++)
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar[a][s >> 2][s & 3],
+                                                                            ac[b][s >> 2][s & 3], acc[a][b], 0, 0, 0);
+            } else if constexpr (std::is_same<T, uint16_t>::value) {
+                u16x8 ar[TR], ac[TC];
+#pragma unroll
+                for (int a = 0; a < TR; a++) ar[a] = *(const u16x8 *)(Lr + (roff + a * 32 + l32) * PITCH + g + hoff);
+#pragma unroll
+                for (int b = 0; b < TC; b++) ac[b] = *(const u16x8 *)(Lc + (coff + b * 32 + l32) * PITCH + g + hoff);
+#pragma unroll
+                for (int a = 0; a < TR; a++)
+#pragma unroll
+                    for (int b = 0; b < TC; b++)
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            __builtin_bit_cast(bf16x8, ar[a]), __builtin_bit_cast(bf16x8, ac[b]), acc[a][b], 0, 0, 0);
+            } else {
+                i32x8 ar[TR], ac[TC];
+#pragma unroll
+                for (int a = 0; a < TR; a++) {
+                    const uint8_t *s = Lr + (roff + a * 32 + l32) * PITCH + g + hoff;
+                    const i32x4 lo = *(const i32x4 *)s, hi = *(const i32x4 *)(s + 16);
+                    ar[a] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+                }
+#pragma unroll
+                for (int b = 0; b < TC; b++) {
+                    const uint8_t *s = Lc + (coff + b * 32 + l32) * PITCH + g + hoff;
+                    const i32x4 lo = *(const i32x4 *)s, hi = *(const i32x4 *)(s + 16);
+                    ac[b] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+                }
+                // operand formats 0/0 = fp8 e4m3 for A and B, E8M0 block scales 127 (= 1.0);
+                // the per-tensor scales are applied in the epilogue
+#pragma unroll
+                for (int a = 0; a < TR; a++)
+#pragma unroll
+                    for (int b = 0; b < TC; b++)
+                        acc[a][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(ar[a], ac[b], acc[a][b], 0, 0, 0,
+                                                                                    127, 0, 127);
+            }
+        }
+    };
+
+    if (kt_begin < kt_end) {
+        if constexpr (!WG) it_set(kt_begin);
+        load_tile(kt_begin);
+        store_tile(0);
+        __syncthreads();
+        for (int kt = kt_begin; kt < kt_end; ++kt) {
+            const int cur = (kt - kt_begin) & 1;
+            const bool has_next = kt + 1 < kt_end;
+            if (has_next) {
+                if constexpr (!WG) it_next();
+                load_tile(kt + 1);
+            }
+            compute_tile(cur);
+            if (has_next) store_tile(cur ^ 1);
+            __syncthreads();
+        }
+    }
+
+    // ---- write-back ----
+    const int ncls = p.ncls_h * p.ncls_w;
+#pragma unroll
+    for (int a = 0; a < TR; a++)
+#pragma unroll
+        for (int b = 0; b < TC; b++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int ir = roff + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                const int ic = coff + b * 32 + (lane & 31);
+                const int ml = SW ? ic : ir, nl = SW ? ir : ic;
+                int m;
+                const int n = nt * BN + nl;
+                bool ok;
+                if constexpr (WG) {
+                    const int c = wg_c0 + ml;
+                    m = wg_tap * p.C + c;
+                    ok = c < p.C && n < p.KN;
+                } else {
+                    m = mt * BM + ml;
+                    ok = m < x.Mrows && n < (DG ? p.C : p.KN);
+                }
+                if (!ok) continue;
+                if (p.splits > 1)
+                    p.ws[(((long long)split * ncls + cls) * p.M + m) * p.N + n] = acc[a][b][v];
+                else
+                    store_out<OP>(p, x, m, n, acc[a][b][v]);
+            }
+}
+
+// ---- instantiation helpers for the per-type launch files ----
+
+// FP32 and BF16 serve all six operations; FP8 serves the forward pass.
+template <typename T> constexpr bool op_enabled(int op) {
+    return !std::is_same<T, uint8_t>::value || op == CG_FWD_NHWC || op == CG_FWD_NCHW;
+}
+// Does the VEC flag change any loader of this operation?
+template <typename T> constexpr bool vec_relevant(int op) {
+    return op == CG_FWD_NHWC || (op == CG_FWD_NCHW && fwd_b_kmajor<T>()) || op == CG_DGRAD_NHWC ||
+           op == CG_DGRAD_NCHW;
+}
+
+template <typename T, int BM, int BN, int BK, int WM, int WN, int OP, bool VEC>
+inline int launch_igemm(const cg_kparams *kp, hipStream_t st) {
+    const long long tiles = (long long)kp->mtiles * kp->ntiles;
+    const long long nb = tiles * kp->splits * kp->ncls_h * kp->ncls_w;
+    hipLaunchKernelGGL((igemm_kernel<T, BM, BN, BK, WM, WN, OP, VEC>), dim3((unsigned)nb), dim3(NT), 0, st, *kp);
+    return (int)hipGetLastError();
+}
+
+// NV: this configuration also has element-load (non-VEC) instantiations.
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool NV, int OP>
+inline int launch_op(const cg_kparams *kp, hipStream_t st) {
+    if constexpr (!op_enabled<T>(OP)) {
+        return (int)hipErrorInvalidValue;
+    } else if constexpr (!vec_relevant<T>(OP)) {
+        return launch_igemm<T, BM, BN, BK, WM, WN, OP, true>(kp, st);
+    } else {
+        if (kp->vec) return launch_igemm<T, BM, BN, BK, WM, WN, OP, true>(kp, st);
+        if constexpr (NV) return launch_igemm<T, BM, BN, BK, WM, WN, OP, false>(kp, st);
+        return (int)hipErrorInvalidValue;
+    }
+}
+
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool NV>
+inline int launch_cfg(const cg_kparams *kp, hipStream_t st) {
+    switch (kp->op) {
+    case CG_FWD_NHWC: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_FWD_NHWC>(kp, st);
+    case CG_FWD_NCHW: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_FWD_NCHW>(kp, st);
+    case CG_WGRAD_NHWC: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_WGRAD_NHWC>(kp, st);
+    case CG_WGRAD_NCHW: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_WGRAD_NCHW>(kp, st);
+    case CG_DGRAD_NHWC: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_DGRAD_NHWC>(kp, st);
+    case CG_DGRAD_NCHW: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_DGRAD_NCHW>(kp, st);
+    }
+    return (int)hipErrorInvalidValue;
+}
+
+template <typename T>
+inline int supports(const cg_tilecfg *table, const bool *novec, int n, int cfg, int op, int vec) {
+    if (cfg < 0 || cfg >= n || op < 0 || op >= CG_NUM_OPKINDS) return 0;
+    const bool en[6] = {op_enabled<T>(0), op_enabled<T>(1), op_enabled<T>(2),
+                        op_enabled<T>(3), op_enabled<T>(4), op_enabled<T>(5)};
+    const bool vr[6] = {vec_relevant<T>(0), vec_relevant<T>(1), vec_relevant<T>(2),
+                        vec_relevant<T>(3), vec_relevant<T>(4), vec_relevant<T>(5)};
+    if (!en[op]) return 0;
+    if (!vec && vr[op] && !novec[cfg]) return 0;
+    (void)table;
+    return 1;
+}
+
+}  // namespace cg

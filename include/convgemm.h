@@ -146,6 +146,14 @@ convgemm_status_t convgemm_conv_ex(const convgemm_desc_t *desc, float alpha, flo
                                    const float *scales,
                                    void *workspace, size_t ws_bytes, convgemm_stream_t stream);
 
+/* Convert n FP32 values on the device to `dtype`, asynchronously on `stream`.
+ * CONVGEMM_BF16: round to nearest even, *scale_out = 1.
+ * CONVGEMM_FP8_E4M3: per-tensor amax scaling, scale = amax/448 (1 if amax is
+ *   0), stored = e4m3fn(x/scale) rounded to nearest even; *scale_out receives
+ *   the scale (the call then waits for the stream).  scale_out may be NULL. */
+convgemm_status_t convgemm_quantize(const float *src, void *dst, size_t n, convgemm_dtype_t dtype,
+                                    float *scale_out, convgemm_stream_t stream);
+
 /* Device workspace bytes convgemm_conv_ex needs for this problem. */
 size_t convgemm_workspace_size(const convgemm_desc_t *desc, convgemm_dtype_t dtype);
 
