@@ -203,6 +203,8 @@ def main(argv=None):
     ap.add_argument("--no-error", action="store_true", help="skip the FP64 error check")
     ap.add_argument("--no-tune", action="store_true", help="use the analytic planner instead of autotuning at setup")
     ap.add_argument("--no-precision", action="store_true", help="skip the BF16/FP8 precision sweep")
+    ap.add_argument("--only", default="", help="comma-separated layer names (e.g. L2_3x3_64to64_56_NHWC) "
+                    "to run alone, for profiling; the JSON value then covers only those")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build targets one GPU; --gpus {a.gpus} is not supported "
@@ -223,6 +225,11 @@ def main(argv=None):
             desc = cg.make_desc(cg.FWD, layout, b, h, w, c, kn, kh, kw, pad, pad, st, st)
             convs.append(dict(name=f"{name}_{'NHWC' if layout == 0 else 'NCHW'}", layout=layout, spec=spec,
                               desc=desc, d=d, flops=layer_flops(spec), bytes=layer_bytes(spec)))
+
+    if a.only:
+        keep = set(a.only.split(","))
+        convs = [cv for cv in convs if cv["name"] in keep]
+        assert convs, f"--only matched nothing: {a.only}"
 
     def run(cv):
         d = cv["d"]

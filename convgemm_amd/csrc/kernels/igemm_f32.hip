@@ -15,10 +15,14 @@ const cg_tilecfg kCfg[] = {
     {128, 128, 16, 2, 2},  // 5
     {64, 64, 16, 2, 2},  // 6
     {64, 256, 32, 1, 4},  // 7
+    {256, 64, 16, 4, 1},  // 8
+    {64, 256, 16, 1, 4},  // 9
+    {256, 128, 16, 2, 2},  // 10
+    {128, 64, 16, 2, 2},  // 11
 };
 constexpr int kNum = sizeof(kCfg) / sizeof(kCfg[0]);
 // configurations that also have element-load (non-16-byte) instantiations
-const bool kNoVec[kNum] = {true, false, false, true, false, false, true, false};
+const bool kNoVec[kNum] = {true, false, false, true, false, false, true, false, false, false, false, false};
 
 }  // namespace
 
@@ -43,6 +47,10 @@ extern "C" int cg_f32_launch(int cfg, const cg_kparams *kp, void *stream) {
     case 5: return launch_cfg<float, 128, 128, 16, 2, 2, false>(kp, st);
     case 6: return launch_cfg<float, 64, 64, 16, 2, 2, true>(kp, st);
     case 7: return launch_cfg<float, 64, 256, 32, 1, 4, false>(kp, st);
+    case 8: return launch_cfg<float, 256, 64, 16, 4, 1, false>(kp, st);
+    case 9: return launch_cfg<float, 64, 256, 16, 1, 4, false>(kp, st);
+    case 10: return launch_cfg<float, 256, 128, 16, 2, 2, false>(kp, st);
+    case 11: return launch_cfg<float, 128, 64, 16, 2, 2, false>(kp, st);
     }
     return (int)hipErrorInvalidValue;
 }

@@ -189,11 +189,13 @@ __global__ void __launch_bounds__(NT) igemm_kernel(const cg_kparams p) {
         b_row[i] = r < (DG ? p.C : p.KN) ? r : -1;
     }
 
-    V ra[PA], rb[PB];
+    // Two register stages: the loads of K tile t+2 are in flight while tile t
+    // is multiplied out of LDS and tile t+1 is written to the other LDS buffer.
+    V ra0[PA], rb0[PB], ra1[PA], rb1[PB];
 
     // reduction iterator (FWD/DGRAD): tile kt -> (tap index ti, channel chunk ch)
     int it_ch = 0, it_ti = 0, it_kx = 0, it_ky = 0, it_vx = 0, it_vy = 0;
-    auto it_set = [&](int kt) {
+    auto it_set = [&](int kt) __attribute__((always_inline)) {
         it_ti = kt / p.kchunks;
         it_ch = kt - it_ti * p.kchunks;
         if constexpr (DG) {
@@ -208,7 +210,7 @@ __global__ void __launch_bounds__(NT) igemm_kernel(const cg_kparams p) {
             it_ky = it_ti - it_kx * p.KW;
         }
     };
-    auto it_next = [&]() {
+    auto it_next = [&]() __attribute__((always_inline)) {
         if (++it_ch == p.kchunks) {
             it_ch = 0;
             ++it_ti;
@@ -223,7 +225,7 @@ __global__ void __launch_bounds__(NT) igemm_kernel(const cg_kparams p) {
     };
 
     // ---- gather one K tile (global -> registers) ----
-    auto load_tile = [&](int kt) {
+    auto load_tile = [&](int kt, V (&ra)[PA], V (&rb)[PB]) __attribute__((always_inline)) {
         if constexpr (!WG) {
             const int r0 = it_ch * BK;
             const int kx = it_kx, ky = it_ky;
@@ -257,26 +259,30 @@ __global__ void __launch_bounds__(NT) igemm_kernel(const cg_kparams p) {
             } else {  // NCHW: lanes walk pixels, one plane per channel
                 const int ih = a_h[0] + dh, iw = a_w[0] + dw;
                 const bool okp = (unsigned)ih < (unsigned)lim_h && (unsigned)iw < (unsigned)lim_w;
-                const unsigned plane = (unsigned)lim_h * lim_w;
-                const unsigned base = (unsigned)a_b[0] * cred * plane + (unsigned)ih * lim_w + (unsigned)iw;
+                const unsigned plane = (unsigned)lim_h * lim_w * ES;  // bytes between channels
+                const unsigned base =
+                    ((unsigned)a_b[0] * cred * lim_h * lim_w + (unsigned)ih * lim_w + (unsigned)iw) * ES;
 #pragma unroll
                 for (int i = 0; i < PA; i++) {
                     const int c = r0 + ld_kq<AK, BM, KQ>(tid, i) * VW;
+                    const unsigned cb = base + (unsigned)c * plane;
 #pragma unroll
                     for (int j = 0; j < VW; j++)
-                        ra[i][j] = ld_one<T>(rA, (okp && c + j < cred) ? (base + (unsigned)(c + j) * plane) * ES : OOB);
+                        ra[i][j] = ld_one<T>(rA, (okp && c + j < cred) ? cb + j * plane : OOB);
                 }
             }
             // B operand
             if constexpr (FW && !fwd_b_kmajor<T>()) {  // [c][tap][kn]: lanes walk kn
                 const int n = b_row[0];
+                const unsigned cstride = (unsigned)p.taps * p.KN * ES;  // bytes between channels
+                const unsigned nbase = ((unsigned)tap * p.KN + (unsigned)n) * ES;
 #pragma unroll
                 for (int i = 0; i < PB; i++) {
                     const int c = r0 + ld_kq<BKIND, BN, KQ>(tid, i) * VW;
+                    const unsigned base = (unsigned)c * cstride + nbase;
 #pragma unroll
                     for (int j = 0; j < VW; j++)
-                        rb[i][j] = ld_one<T>(rB, (n >= 0 && c + j < p.C)
-                                                     ? (((unsigned)(c + j) * p.taps + tap) * p.KN + n) * ES : OOB);
+                        rb[i][j] = ld_one<T>(rB, (n >= 0 && c + j < p.C) ? base + j * cstride : OOB);
                 }
             } else {  // FWD LP: [kn][tap][c];  DGRAD: [c][tap][kn] -- contiguous along the reduction
                 const int k0 = r0 + ld_kq<BKIND, BN, KQ>(tid, 0) * VW;
@@ -366,7 +372,7 @@ __global__ void __launch_bounds__(NT) igemm_kernel(const cg_kparams p) {
         }
     };
 
-    auto store_tile = [&](int buf) {
+    auto store_tile = [&](int buf, const V (&ra)[PA], const V (&rb)[PB]) __attribute__((always_inline)) {
         T *dA = sA + buf * BM * PITCH;
         T *dB = sB + buf * BN * PITCH;
 #pragma unroll
@@ -389,7 +395,7 @@ __global__ void __launch_bounds__(NT) igemm_kernel(const cg_kparams p) {
 #pragma unroll
             for (int v = 0; v < 16; v++) acc[a][b][v] = 0.f;
 
-    auto compute_tile = [&](int buf) {
+    auto compute_tile = [&](int buf) __attribute__((always_inline)) {
         const T *Lr = SW ? sB + buf * BN * PITCH : sA + buf * BM * PITCH;
         const T *Lc = SW ? sA + buf * BM * PITCH : sB + buf * BN * PITCH;
         const int l32 = lane & 31;
@@ -459,21 +465,36 @@ __global__ void __launch_bounds__(NT) igemm_kernel(const cg_kparams p) {
         }
     };
 
+    // Tile i (counted from kt_begin) lives in register stage i&1 and LDS buffer
+    // i&1.  At step i: issue the loads of tile i+2 into stage i&1 (stored to
+    // LDS at step i-1), multiply LDS buffer i&1, write stage (i+1)&1 into LDS
+    // buffer (i+1)&1 (last read at step i-1, before the barrier), barrier.
+    auto step = [&](auto stage, int kt) __attribute__((always_inline)) {
+        constexpr int S = decltype(stage)::value;
+        if (kt + 2 < kt_end) {
+            if constexpr (!WG) it_next();
+            if constexpr (S == 0) load_tile(kt + 2, ra0, rb0);
+            else load_tile(kt + 2, ra1, rb1);
+        }
+        compute_tile(S);
+        if (kt + 1 < kt_end) {
+            if constexpr (S == 0) store_tile(1, ra1, rb1);
+            else store_tile(0, ra0, rb0);
+        }
+        __syncthreads();
+    };
     if (kt_begin < kt_end) {
         if constexpr (!WG) it_set(kt_begin);
-        load_tile(kt_begin);
-        store_tile(0);
+        load_tile(kt_begin, ra0, rb0);
+        if (kt_begin + 1 < kt_end) {
+            if constexpr (!WG) it_next();
+            load_tile(kt_begin + 1, ra1, rb1);
+        }
+        store_tile(0, ra0, rb0);
         __syncthreads();
-        for (int kt = kt_begin; kt < kt_end; ++kt) {
-            const int cur = (kt - kt_begin) & 1;
-            const bool has_next = kt + 1 < kt_end;
-            if (has_next) {
-                if constexpr (!WG) it_next();
-                load_tile(kt + 1);
-            }
-            compute_tile(cur);
-            if (has_next) store_tile(cur ^ 1);
-            __syncthreads();
+        for (int kt = kt_begin; kt < kt_end; kt += 2) {
+            step(std::integral_constant<int, 0>{}, kt);
+            if (kt + 1 < kt_end) step(std::integral_constant<int, 1>{}, kt + 1);
         }
     }
 
@@ -563,7 +584,8 @@ inline int supports(const cg_tilecfg *table, const bool *novec, int n, int cfg, 
                         vec_relevant<T>(3), vec_relevant<T>(4), vec_relevant<T>(5)};
     if (!en[op]) return 0;
     if (!vec && vr[op] && !novec[cfg]) return 0;
-    (void)table;
+    // tiles of 256x128 and up spill registers in the backward-data gather
+    if ((op == CG_DGRAD_NHWC || op == CG_DGRAD_NCHW) && table[cfg].BM * table[cfg].BN >= 256 * 128) return 0;
     return 1;
 }
 

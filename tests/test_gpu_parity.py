@@ -98,9 +98,10 @@ CFG_SHAPES = [Shape(2, 11, 10, 24, 40, 3, 3, 1, 1, 1, 1, 1, 1), Shape(1, 13, 12,
 
 
 @pytest.mark.parametrize("splits", [1, 3])
-@pytest.mark.parametrize("cfg", range(9))
+@pytest.mark.parametrize("cfg", range(12))
 def test_every_tile_configuration(cfg, splits):
-    """Force each kernel instantiation (and the split-K reduction) on every op."""
+    """Force each kernel instantiation (and the split-K reduction) on every op.
+    A configuration an operation does not offer falls back to the planner."""
     cg.force_config(cfg, splits)
     try:
         for si, s in enumerate(CFG_SHAPES):
