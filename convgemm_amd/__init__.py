@@ -129,6 +129,22 @@ def lib() -> ctypes.CDLL:
     L.convgemm_tune_clear.restype = None
     L.convgemm_plan.argtypes = [ctypes.POINTER(Desc), ci, ctypes.POINTER(ci), ctypes.POINTER(ci)]
     L.convgemm_plan.restype = ci
+    L.convgemm_costmodel_train.argtypes = [ci, cf, ctypes.c_uint, ctypes.POINTER(cf), vp]
+    L.convgemm_costmodel_train.restype = ci
+    L.convgemm_costmodel_save.argtypes = [ctypes.c_char_p]
+    L.convgemm_costmodel_save.restype = ci
+    L.convgemm_costmodel_load.argtypes = [ctypes.c_char_p]
+    L.convgemm_costmodel_load.restype = ci
+    L.convgemm_costmodel_predict.argtypes = [ctypes.POINTER(Desc), ci, ctypes.POINTER(ci), ctypes.POINTER(ci),
+                                             ctypes.POINTER(cf), ctypes.POINTER(ci)]
+    L.convgemm_costmodel_predict.restype = ci
+    L.cg_tune_get.argtypes = [ctypes.POINTER(Desc), ci, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(cf), ci]
+    L.cg_tune_get.restype = ci
+    L.cg_mlp_loss_grad.argtypes = [ctypes.POINTER(cf), ctypes.POINTER(cf), ctypes.POINTER(cf), ci,
+                                   ctypes.POINTER(cf), ctypes.POINTER(cf)]
+    L.cg_mlp_loss_grad.restype = ci
+    L.cg_mlp_dims.argtypes = [ctypes.POINTER(ci)] * 3
+    L.cg_mlp_dims.restype = ci
     L.cg_force_config.argtypes = [ci, ci]
     L.cg_force_config.restype = None
     L.cg_last_plan.argtypes = [ctypes.POINTER(ci)] * 3
@@ -320,6 +336,65 @@ def tune_load(path: str) -> None:
 
 def tune_clear() -> None:
     lib().convgemm_tune_clear()
+
+
+def tune_get(desc: Desc, dtype=F32):
+    """Recorded timings of a tuned problem: list of (cfg, splits, microseconds)."""
+    L = lib()
+    c, s, u = (ctypes.c_int * 256)(), (ctypes.c_int * 256)(), (ctypes.c_float * 256)()
+    n = L.cg_tune_get(ctypes.byref(desc), int(dtype), c, s, u, 256)
+    return [(c[i], s[i], u[i]) for i in range(n)]
+
+
+def costmodel_train(epochs=1500, lr=3e-3, seed=0, stream=None) -> float:
+    """Train the cost model on every timing in the tuning database (device kernels); returns the final loss."""
+    L = lib()
+    loss = ctypes.c_float()
+    st = L.convgemm_costmodel_train(int(epochs), float(lr), int(seed), ctypes.byref(loss), _stream_handle(stream))
+    if st != 0:
+        raise ConvGemmError(st, L.convgemm_last_error_message().decode())
+    return loss.value
+
+
+def costmodel_save(path: str) -> None:
+    if lib().convgemm_costmodel_save(path.encode()) != 0:
+        raise ConvGemmError(1, lib().convgemm_last_error_message().decode())
+
+
+def costmodel_load(path: str) -> None:
+    if lib().convgemm_costmodel_load(path.encode()) != 0:
+        raise ConvGemmError(1, lib().convgemm_last_error_message().decode())
+
+
+def costmodel_predict(desc: Desc, dtype=F32):
+    """Predicted microseconds of every candidate: list of (cfg, splits, us)."""
+    L = lib()
+    c, s, u = (ctypes.c_int * 256)(), (ctypes.c_int * 256)(), (ctypes.c_float * 256)()
+    n = ctypes.c_int(256)
+    st = L.convgemm_costmodel_predict(ctypes.byref(desc), int(dtype), c, s, u, ctypes.byref(n))
+    if st != 0:
+        raise ConvGemmError(st, L.convgemm_last_error_message().decode())
+    return [(c[i], s[i], u[i]) for i in range(n.value)]
+
+
+def mlp_loss_grad(params, X, T):
+    """Loss and gradients of the cost-model MLP computed by the device kernels (test hook)."""
+    import numpy as np
+    L = lib()
+    F, H, NP = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    L.cg_mlp_dims(ctypes.byref(F), ctypes.byref(H), ctypes.byref(NP))
+    p = np.ascontiguousarray(params, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    T = np.ascontiguousarray(T, np.float32)
+    assert p.size == NP.value and X.shape[1] == F.value and X.shape[0] == T.size
+    g = np.empty(NP.value, np.float32)
+    loss = ctypes.c_float()
+    fp = ctypes.POINTER(ctypes.c_float)
+    rc = L.cg_mlp_loss_grad(p.ctypes.data_as(fp), X.ctypes.data_as(fp), T.ctypes.data_as(fp), int(T.size),
+                            g.ctypes.data_as(fp), ctypes.byref(loss))
+    if rc != 0:
+        raise ConvGemmError(3, "mlp kernels failed")
+    return float(loss.value), g
 
 
 def last_plan():

@@ -274,6 +274,29 @@ convgemm_status_t convgemm_autotune(const convgemm_desc_t *desc, convgemm_dtype_
     return cg_set_error(CONVGEMM_SUCCESS, "");
 }
 
+/* All recorded timings of one problem (prepared as for aligned operands).
+ * Returns the number of candidates written (0 if the problem was not tuned). */
+int cg_tune_get(const convgemm_desc_t *desc, convgemm_dtype_t dtype, int *cfgs, int *splits, float *us, int max) {
+    cg_kparams kp;
+    int elem, chunk;
+    if (cg_prepare(desc, dtype, (void *)256, (void *)256, NULL, &kp, &elem, &chunk) != CONVGEMM_SUCCESS) return 0;
+    int k[KEYLEN];
+    cg_key_of(elem, &kp, k);
+    int n = 0;
+    pthread_mutex_lock(&t_lock);
+    if (t_cap) {
+        tentry *e = find_slot(t_tab, t_cap, k);
+        if (e->used)
+            for (int i = 0; i < e->n && n < max; i++, n++) {
+                cfgs[n] = e->cfg[i];
+                splits[n] = e->splits[i];
+                us[n] = e->us[i];
+            }
+    }
+    pthread_mutex_unlock(&t_lock);
+    return n;
+}
+
 /* ------------------------------------------------------------ persistence */
 
 /* Text format, one problem per line:

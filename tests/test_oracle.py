@@ -304,3 +304,23 @@ def test_sampled_oracle_matches_direct(layout, si):
     xs = np.stack([RNG.integers(0, n, 40) for n in (s.b, s.h, s.w, s.c)], 1)
     np.testing.assert_allclose(sampled.dgrad_sampled(layout, s, W, dY, xs), sampled.nhwc_index(layout, dX, xs),
                                rtol=1e-12, atol=1e-12)
+
+
+# ---------------------------------------------------------------- MLP oracle
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    from oracle import mlp
+    rng = np.random.default_rng(12)
+    p = rng.standard_normal(mlp.NP) * 0.2
+    X = rng.standard_normal((17, mlp.F))
+    T = rng.standard_normal(17)
+    L, g = mlp.loss_grad(p, X, T)
+    assert abs(L - mlp.loss(p, X, T)) < 1e-14
+    idx = rng.choice(mlp.NP, 200, replace=False)
+    idx = np.concatenate([idx, [mlp.NP - 1, mlp.F * mlp.H, 0]])  # include b3, b1[0], W1[0,0]
+    h = 1e-6
+    for i in idx:
+        e = np.zeros(mlp.NP)
+        e[i] = h
+        fd = (mlp.loss(p + e, X, T) - mlp.loss(p - e, X, T)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, fd, g[i])
