@@ -194,6 +194,25 @@ def cpu_baseline(budget_s=12.0):
                       f"OpenMP {threads} threads)"}
 
 
+def copy_probe(torch, mib, steps, warmup):
+    """Plain device-to-device copy of `mib` MiB: a kernel whose HBM traffic is known
+    (reads and writes mib MiB each), to scale the FETCH_SIZE/WRITE_SIZE counters."""
+    n = mib * (1 << 20) // 4
+    src = torch.randn(n, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    print(json.dumps({"copy_probe_bytes": n * 4, "steps": steps, "ms": round(ms, 4),
+                      "GBps_read_plus_write": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,6 +222,10 @@ def main(argv=None):
     ap.add_argument("--no-error", action="store_true", help="skip the FP64 error check")
     ap.add_argument("--no-tune", action="store_true", help="use the analytic planner instead of autotuning at setup")
     ap.add_argument("--no-precision", action="store_true", help="skip the BF16/FP8 precision sweep")
+    ap.add_argument("--tune-db", default="", help="tuning database: loaded if present (layers found there are "
+                    "not re-tuned), written back after tuning")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB per step (calibrates HBM byte counters)")
     ap.add_argument("--only", default="", help="comma-separated layer names (e.g. L2_3x3_64to64_56_NHWC) "
                     "to run alone, for profiling; the JSON value then covers only those")
     a = ap.parse_args(argv)
@@ -217,6 +240,10 @@ def main(argv=None):
     assert torch.cuda.is_available(), "bench.py needs an MI355X"
     cg.lib()
     stream = torch.cuda.current_stream()
+    if a.copy_probe:
+        return copy_probe(torch, a.copy_probe, a.steps, a.warmup)
+    if a.tune_db and os.path.exists(a.tune_db):
+        cg.tune_load(a.tune_db)
     convs = []
     for layout in (0, 1):
         for li, (name, spec) in enumerate(LAYERS):
@@ -238,10 +265,16 @@ def main(argv=None):
 
     # plans: the autotuner times every tile configuration / split-K candidate
     # once per layer during setup (untimed), like a benchmark-mode library
+    tuned = 0
     for cv in convs:
-        if not a.no_tune:
+        if not a.no_tune and not cg.tune_get(cv["desc"]):
             d = cv["d"]
             cg.autotune(cv["desc"], d["W"], d["X"], d["Y"], reps=3, stream=stream)
+            tuned += 1
+    if a.tune_db and tuned:
+        cg.tune_save(a.tune_db)
+    if a.no_tune:
+        cg.set_planner(cg.PLAN_ANALYTIC)
     for cv in convs:
         run(cv)
         cv["plan"] = cg.last_plan()
@@ -274,13 +307,15 @@ def main(argv=None):
     single = [cv for cv in convs if cv["plan"][2] == 1] or convs
     dom = max(single, key=lambda cv: cv["ms"])
     achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
+    # HBM bytes per launch of the dominant kernel: FETCH_SIZE/WRITE_SIZE counter
+    # passes scaled by a plain copy (profiles/traffic_fwd_fp32.md), valid while
+    # the layer runs the plan that was measured
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_fwd_fp32.json")
     if os.path.exists(tpath):
-        try:
-            traffic = json.load(open(tpath)).get(dom["name"])
-        except Exception:
-            traffic = None
+        rec = json.load(open(tpath))["layers"].get(dom["name"])
+        if rec and (rec["plan"]["cfg"], rec["plan"]["splits"]) == tuple(dom["plan"][:2]):
+            traffic = rec["traffic_bytes"]
 
     errors = {}
     if not a.no_error:

@@ -102,14 +102,17 @@ __global__ void __launch_bounds__(NT) igemm_kernel(const cg_kparams p) {
 
     // ---- block -> (tile, split, class).  Workgroups are dealt to the 8 XCDs
     // round-robin; give each XCD a contiguous run of tiles so neighbouring
-    // tiles (shared weight panel, shared input halo) meet in one L2.
+    // tiles (shared weight panel, shared input halo) meet in one L2.  Within
+    // the run the N tiles of one M panel are adjacent: the gathered operand
+    // (the large one) is then fetched from HBM once, the small weight panel
+    // stays L2-resident (measured: A re-read ntiles times with M fastest).
     const int nb = gridDim.x, bid = blockIdx.x;
     const int xcd = bid & 7, per = nb >> 3, rem = nb & 7;
     const int lin = xcd * per + min(xcd, rem) + (bid >> 3);
     const int tiles = p.mtiles * p.ntiles;
     const int tile = lin % tiles, rest = lin / tiles;
     const int split = rest % p.splits, cls = rest / p.splits;
-    const int mt = tile % p.mtiles, nt = tile / p.mtiles;
+    const int nt = tile % p.ntiles, mt = tile / p.ntiles;
 
     const Ctx x = make_ctx(p, cls);
     if (!WG && mt * BM >= x.Mrows) return;  // uniform: DGRAD classes smaller than the grid
