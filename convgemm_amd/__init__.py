@@ -45,6 +45,8 @@ EXPORTED_SYMBOLS = (
     "convgemm_set_planner", "convgemm_autotune", "convgemm_tune_count", "convgemm_tune_save", "convgemm_tune_load",
     "convgemm_tune_clear", "convgemm_plan", "convgemm_costmodel_train", "convgemm_costmodel_save",
     "convgemm_costmodel_load", "convgemm_costmodel_predict",
+    # convgemm_batch.h
+    "convgemm_batch_create", "convgemm_batch_run", "convgemm_batch_info", "convgemm_batch_destroy",
 )
 
 PLAN_ANALYTIC, PLAN_TUNED, PLAN_LEARNED, PLAN_MODEL = 0, 1, 2, 3
@@ -65,6 +67,15 @@ class Epilogue(ctypes.Structure):
     _fields_ = [("bias", ctypes.c_void_p), ("mean", ctypes.c_void_p), ("inv_std", ctypes.c_void_p),
                 ("gamma", ctypes.c_void_p), ("beta", ctypes.c_void_p), ("relu", ctypes.c_int)]
 
+
+class Problem(ctypes.Structure):
+    """convgemm_problem_t: one convolution of a batch."""
+    _fields_ = [("desc", Desc), ("alpha", ctypes.c_float), ("beta", ctypes.c_float), ("in_", ctypes.c_void_p),
+                ("x", ctypes.c_void_p), ("out", ctypes.c_void_p), ("epi", Epilogue),
+                ("scales", ctypes.c_float * 2)]
+
+
+BATCH_GRAPH, BATCH_PERSISTENT, BATCH_STATIC_WEIGHTS = 1, 2, 4
 
 _lib = None
 
@@ -145,6 +156,14 @@ def lib() -> ctypes.CDLL:
     L.cg_mlp_loss_grad.restype = ci
     L.cg_mlp_dims.argtypes = [ctypes.POINTER(ci)] * 3
     L.cg_mlp_dims.restype = ci
+    L.convgemm_batch_create.argtypes = [ctypes.POINTER(Problem), ci, ci, ctypes.c_uint, ctypes.POINTER(vp)]
+    L.convgemm_batch_create.restype = ci
+    L.convgemm_batch_run.argtypes = [vp, vp]
+    L.convgemm_batch_run.restype = ci
+    L.convgemm_batch_info.argtypes = [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci)]
+    L.convgemm_batch_info.restype = ci
+    L.convgemm_batch_destroy.argtypes = [vp]
+    L.convgemm_batch_destroy.restype = None
     L.cg_force_config.argtypes = [ci, ci]
     L.cg_force_config.restype = None
     L.cg_last_plan.argtypes = [ctypes.POINTER(ci)] * 3
@@ -407,3 +426,58 @@ def last_plan():
 def force_config(cfg: int = -1, splits: int = 0) -> None:
     """Tuning hook: force tile configuration `cfg` and split-K `splits` on every call (cfg < 0 clears)."""
     lib().cg_force_config(int(cfg), int(splits))
+
+
+def problem(desc: Desc, in_, x, out, alpha=1.0, beta=0.0, bias=None, mean=None, inv_std=None, gamma=None,
+            bnbeta=None, relu=False, scales=None) -> Problem:
+    """One convolution for a Batch (the arguments of conv_ex)."""
+    pr = Problem()
+    pr.desc = desc
+    pr.alpha, pr.beta = float(alpha), float(beta)
+    pr.in_, pr.x, pr.out = _ptr(in_), _ptr(x), _ptr(out)
+    pr.epi = Epilogue(_ptr(bias), _ptr(mean), _ptr(inv_std), _ptr(gamma), _ptr(bnbeta), int(bool(relu)))
+    sc = scales if scales is not None else (1.0, 1.0)
+    pr.scales[0], pr.scales[1] = float(sc[0]), float(sc[1])
+    return pr
+
+
+class Batch:
+    """convgemm_batch_t: many independent convolutions in grouped launches,
+    optionally replayed from a hipGraph (flags BATCH_GRAPH, BATCH_PERSISTENT,
+    BATCH_STATIC_WEIGHTS).  Keep the operand tensors alive while the batch
+    is used: their addresses are bound at creation."""
+
+    def __init__(self, problems, dtype=F32, flags=0, keep=None):
+        L = lib()
+        arr = (Problem * max(len(problems), 1))(*problems)
+        h = ctypes.c_void_p()
+        st = L.convgemm_batch_create(arr, len(problems), int(dtype), int(flags), ctypes.byref(h))
+        if st != 0:
+            raise ConvGemmError(st, L.convgemm_last_error_message().decode())
+        self._h = h
+        self._keep = keep
+
+    def run(self, stream=None) -> None:
+        L = lib()
+        st = L.convgemm_batch_run(self._h, _stream_handle(stream))
+        if st != 0:
+            raise ConvGemmError(st, L.convgemm_last_error_message().decode())
+
+    def info(self):
+        """(grouped implicit-GEMM launches, kernels per run, replays a graph)."""
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        st = lib().convgemm_batch_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        if st != 0:
+            raise ConvGemmError(st, lib().convgemm_last_error_message().decode())
+        return a.value, b.value, bool(c.value)
+
+    def close(self) -> None:
+        if self._h:
+            lib().convgemm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -76,6 +76,30 @@ int cg_tilecfg_supports(int elem, int cfg, int op, int vec);
 int cg_launch_igemm(int elem, int cfg, const cg_kparams *kp, void *stream);
 /* Sum the split-K partials and apply the epilogue / layout store. */
 int cg_launch_splitk_reduce(const cg_kparams *kp, void *stream);
+
+/* A group of independent problems run by one launch.  Work items are
+ * numbered 0..total-1; problem j owns items [start[j], start[j+1]).  Each of
+ * the `grid` workgroups takes items blockIdx.x, blockIdx.x + grid, ...
+ * (grid == total: one item each; grid < total: persistent workgroups).
+ * plist and start are device arrays. */
+typedef struct {
+    const cg_kparams *plist;
+    const int *start;
+    int nprob;
+    int total;
+    int grid;
+} cg_group;
+
+/* Grouped implicit GEMM: every problem shares elem, cfg, op and vec (taken
+ * from kp0, a host copy of any member); items are the problems' workgroups
+ * (tiles x splits x classes). */
+int cg_launch_igemm_grouped(int elem, int cfg, const cg_kparams *kp0, const cg_group *g, void *stream);
+/* Grouped split-K reduction of problems sharing the operation `op`; items
+ * are 256-element blocks, cg_reduce_items() per problem. */
+int cg_launch_splitk_reduce_grouped(int op, const cg_group *g, void *stream);
+long long cg_reduce_items(const cg_kparams *kp);
+/* Resident workgroups per CU of a configuration (occupancy), or 0. */
+int cg_igemm_occupancy(int elem, int cfg, const cg_kparams *kp0);
 /* out[i] = beta*out[i] (0 if beta == 0), n elements. */
 int cg_launch_scale(float *out, long long n, float beta, void *stream);
 /* dst = src with its three dimensions [d0][d1][d2] permuted:

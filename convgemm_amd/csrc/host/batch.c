@@ -1,0 +1,449 @@
+/*
+ * batch.c -- many independent convolutions in a few grouped launches
+ * (include/convgemm_batch.h).
+ *
+ * Creation plans every problem as convgemm_conv_ex would (same validation,
+ * quick-return rules and planner), then sorts the problems by kernel
+ * instance (tile configuration, operation, vector loads) and, inside an
+ * instance, by K tiles per workgroup item, longest first.  Each instance is
+ * one grouped launch over the tiles of all its problems; split-K problems
+ * share one grouped reduction per operation.  Every problem's item range
+ * starts at a multiple of 8 so the kernels can deal each problem's tiles to
+ * the 8 XCDs in contiguous runs.  One device allocation holds the problem
+ * tables, the repacked weights and the split-K partials.
+ */
+#include <limits.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "convgemm.h"
+#include "convgemm_batch.h"
+#include "../cg_internal.h"
+#include "cg_host.h"
+
+#define AL256(x) (((size_t)(x) + 255) & ~(size_t)255)
+
+typedef struct {
+    cg_kparams kp;
+    int cfg;
+    long long items; /* workgroup items of the implicit GEMM */
+    long long cost;  /* K tiles per item: the ordering key */
+    int repack, d0, d1, d2, perm;
+    const void *wsrc;
+    size_t wbytes, pbytes;
+} bprob;
+
+typedef struct {
+    int cfg;
+    cg_kparams kp0; /* op and vec of the group */
+    cg_group g;
+} glaunch;
+
+typedef struct {
+    int op;
+    cg_group g;
+} rlaunch;
+
+typedef struct {
+    const void *src;
+    void *dst;
+    int d0, d1, d2, perm;
+} repack_t;
+
+typedef struct {
+    float *out;
+    long long n;
+    float beta;
+} scale_t;
+
+struct convgemm_batch_s {
+    int elem;
+    unsigned flags;
+    void *dmem;
+    int ngl, nrl, nrep, nsc;
+    glaunch *gl;
+    rlaunch rl[CG_NUM_OPKINDS];
+    repack_t *rep;
+    scale_t *sc;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+static convgemm_status_t errf(convgemm_status_t st, const char *msg) { return cg_set_error(st, msg); }
+
+static int cmp_prob(const void *a, const void *b) {
+    const bprob *x = (const bprob *)a, *y = (const bprob *)b;
+    if (x->cfg != y->cfg) return x->cfg - y->cfg;
+    if (x->kp.op != y->kp.op) return x->kp.op - y->kp.op;
+    if (x->kp.vec != y->kp.vec) return x->kp.vec - y->kp.vec;
+    if (x->cost != y->cost) return x->cost > y->cost ? -1 : 1;
+    return 0;
+}
+
+static int cmp_red(const void *a, const void *b) {
+    const cg_kparams *x = (const cg_kparams *)a, *y = (const cg_kparams *)b;
+    return x->op - y->op;
+}
+
+/* Plan one problem.  *kind: 1 = implicit GEMM (bp filled), 2 = beta scaling
+ * only (sc filled, alpha == 0), 0 = no-op. */
+static convgemm_status_t plan_one(const convgemm_problem_t *q, convgemm_dtype_t dtype, bprob *bp, scale_t *sc,
+                                  int *kind) {
+    const convgemm_desc_t *d = &q->desc;
+    cg_kparams kp;
+    int elem, chunk;
+    *kind = 0;
+    convgemm_status_t st = cg_prepare(d, dtype, q->in, q->x, NULL, &kp, &elem, &chunk);
+    if (st != CONVGEMM_SUCCESS) return st;
+    if (chunk < d->b)
+        return errf(CONVGEMM_ERR_UNSUPPORTED, "a batch problem exceeds the 2 GiB addressing window");
+    const int es = cg_esize(elem);
+    const long long P = (long long)d->b * kp.Ho * kp.Wo;
+    const long long K = (long long)d->c * d->kh * d->kw;
+    long long gm, gn, gk, out_elems;
+    if (d->op == CONVGEMM_FWD) {
+        gm = d->kn; gn = P; gk = K; out_elems = P * d->kn;
+    } else if (d->op == CONVGEMM_WGRAD) {
+        gm = d->kn; gn = K; gk = P; out_elems = K * d->kn;
+    } else {
+        gm = K; gn = P; gk = d->kn; out_elems = (long long)d->b * d->h * d->w * d->c;
+    }
+    /* the quick-return rules of a single call (src/gemm_blis_B3A2C0_orig.c:45-47, src/quick_gemm.h) */
+    if (gm == 0 || gn == 0) return CONVGEMM_SUCCESS;
+    if ((q->alpha == 0.0f || gk == 0) && q->beta == 1.0f) return CONVGEMM_SUCCESS;
+    if (!q->out) return errf(CONVGEMM_ERR_INVALID_VALUE, "a batch problem has a NULL output pointer");
+    if (q->alpha == 0.0f) {
+        sc->out = (float *)q->out;
+        sc->n = out_elems;
+        sc->beta = q->beta;
+        *kind = 2;
+        return CONVGEMM_SUCCESS;
+    }
+    if (gk == 0) return CONVGEMM_SUCCESS;
+    if (!q->in || !q->x) return errf(CONVGEMM_ERR_INVALID_VALUE, "a batch problem has a NULL input pointer");
+    if (q->epi.mean && (!q->epi.inv_std || !q->epi.gamma || !q->epi.beta))
+        return errf(CONVGEMM_ERR_INVALID_VALUE, "batch norm needs inv_std, gamma and beta");
+
+    memset(bp, 0, sizeof(*bp));
+    kp.out = (float *)q->out;
+    kp.alpha = q->alpha;
+    kp.beta = q->beta;
+    kp.scale = elem == CG_ELEM_FP8 ? q->scales[0] * q->scales[1] : 1.0f;
+    if (d->op == CONVGEMM_FWD) {
+        kp.bias = q->epi.bias; kp.mean = q->epi.mean; kp.inv_std = q->epi.inv_std;
+        kp.gamma = q->epi.gamma; kp.bnb = q->epi.beta; kp.relu = q->epi.relu != 0;
+    }
+    kp.A = q->x;
+    kp.a_bytes = d->op == CONVGEMM_DGRAD ? P * d->kn * es : (long long)d->b * d->h * d->w * d->c * es;
+    kp.Bm = q->in;
+    kp.b_bytes = d->op == CONVGEMM_WGRAD ? P * d->kn * es : K * d->kn * es;
+    bp->repack = cg_weight_repack(d, elem, &bp->d0, &bp->d1, &bp->d2, &bp->perm);
+    bp->wsrc = q->in;
+    bp->wbytes = bp->repack ? (size_t)(K * d->kn * es) : 0;
+
+    int cfg, splits;
+    cg_choose_plan(elem, &kp, &cfg, &splits);
+    cg_tilecfg t;
+    if (cfg < 0 || cg_get_tilecfg(elem, cfg, &t))
+        return errf(CONVGEMM_ERR_UNSUPPORTED, "no kernel configuration for a batch problem");
+    cg_fill_tiles(&kp, &t, splits);
+    bp->kp = kp;
+    bp->cfg = cfg;
+    bp->pbytes = cg_partial_bytes(&kp);
+    bp->items = (long long)kp.mtiles * kp.ntiles * kp.splits * kp.ncls_h * kp.ncls_w;
+    bp->cost = (kp.KT + kp.splits - 1) / kp.splits;
+    *kind = 1;
+    return CONVGEMM_SUCCESS;
+}
+
+static convgemm_status_t issue(convgemm_batch_t bt, hipStream_t s, int with_repack) {
+    int e;
+    for (int i = 0; i < bt->nsc; i++)
+        if ((e = cg_launch_scale(bt->sc[i].out, bt->sc[i].n, bt->sc[i].beta, s))) return cg_hip_fail(e, "scale");
+    if (with_repack)
+        for (int i = 0; i < bt->nrep; i++) {
+            const repack_t *r = &bt->rep[i];
+            if ((e = cg_launch_permute3(r->src, r->dst, r->d0, r->d1, r->d2, r->perm, cg_esize(bt->elem), s)))
+                return cg_hip_fail(e, "weight repack");
+        }
+    for (int i = 0; i < bt->ngl; i++)
+        if ((e = cg_launch_igemm_grouped(bt->elem, bt->gl[i].cfg, &bt->gl[i].kp0, &bt->gl[i].g, s)))
+            return cg_hip_fail(e, "grouped implicit GEMM");
+    for (int i = 0; i < bt->nrl; i++)
+        if ((e = cg_launch_splitk_reduce_grouped(bt->rl[i].op, &bt->rl[i].g, s)))
+            return cg_hip_fail(e, "grouped split-K reduction");
+    return CONVGEMM_SUCCESS;
+}
+
+static void release(convgemm_batch_t bt) {
+    if (!bt) return;
+    if (bt->exec) hipGraphExecDestroy(bt->exec);
+    if (bt->graph) hipGraphDestroy(bt->graph);
+    if (bt->dmem) hipFree(bt->dmem);
+    free(bt->gl);
+    free(bt->rep);
+    free(bt->sc);
+    free(bt);
+}
+
+convgemm_status_t convgemm_batch_create(const convgemm_problem_t *problems, int n, convgemm_dtype_t dtype,
+                                        unsigned flags, convgemm_batch_t *batch) {
+    if (!batch) return errf(CONVGEMM_ERR_INVALID_VALUE, "batch output pointer is NULL");
+    *batch = NULL;
+    if (n < 0 || (n > 0 && !problems)) return errf(CONVGEMM_ERR_INVALID_VALUE, "invalid problem list");
+    if (flags & ~(unsigned)(CONVGEMM_BATCH_GRAPH | CONVGEMM_BATCH_PERSISTENT | CONVGEMM_BATCH_STATIC_WEIGHTS))
+        return errf(CONVGEMM_ERR_INVALID_VALUE, "unknown batch flags");
+    if (dtype != CONVGEMM_F32 && dtype != CONVGEMM_BF16 && dtype != CONVGEMM_FP8_E4M3)
+        return errf(CONVGEMM_ERR_INVALID_VALUE, "unknown dtype");
+
+    convgemm_batch_t bt = (convgemm_batch_t)calloc(1, sizeof(*bt));
+    bprob *bp = (bprob *)calloc((size_t)(n > 0 ? n : 1), sizeof(bprob));
+    int *hstart = NULL;
+    cg_kparams *htab = NULL;
+    convgemm_status_t st = CONVGEMM_SUCCESS;
+    if (!bt || !bp) {
+        st = errf(CONVGEMM_ERR_ALLOC, "host allocation failed");
+        goto fail;
+    }
+    bt->elem = dtype == CONVGEMM_BF16 ? CG_ELEM_BF16 : dtype == CONVGEMM_FP8_E4M3 ? CG_ELEM_FP8 : CG_ELEM_F32;
+    bt->flags = flags;
+    bt->sc = (scale_t *)calloc((size_t)(n > 0 ? n : 1), sizeof(scale_t));
+    if (!bt->sc) {
+        st = errf(CONVGEMM_ERR_ALLOC, "host allocation failed");
+        goto fail;
+    }
+
+    /* ---- plan every problem ---- */
+    int np = 0;
+    for (int i = 0; i < n; i++) {
+        int kind;
+        st = plan_one(&problems[i], dtype, &bp[np], &bt->sc[bt->nsc], &kind);
+        if (st != CONVGEMM_SUCCESS) goto fail;
+        if (kind == 1) np++;
+        else if (kind == 2) bt->nsc++;
+    }
+    qsort(bp, (size_t)np, sizeof(bprob), cmp_prob);
+
+    /* ---- groups, item ranges, device layout ---- */
+    int ngl = 0, nsplit = 0, nrep = 0;
+    for (int i = 0; i < np; i++) {
+        if (bp[i].kp.splits > 1) nsplit++;
+        if (bp[i].repack) nrep++;
+    }
+    for (int i = 0; i < np; i++)
+        if (i == 0 || bp[i - 1].cfg != bp[i].cfg || bp[i - 1].kp.op != bp[i].kp.op ||
+            bp[i - 1].kp.vec != bp[i].kp.vec)
+            ngl++;
+    int nops = 0;
+    {
+        int seen[CG_NUM_OPKINDS] = {0};
+        for (int i = 0; i < np; i++)
+            if (bp[i].kp.splits > 1 && !seen[bp[i].kp.op]) { seen[bp[i].kp.op] = 1; nops++; }
+    }
+    const size_t tab_bytes = AL256(sizeof(cg_kparams) * (size_t)(np + nsplit));
+    const size_t start_bytes = AL256(sizeof(int) * (size_t)(np + ngl + nsplit + nops));
+    size_t wtot = 0, ptot = 0;
+    for (int i = 0; i < np; i++) {
+        wtot += AL256(bp[i].wbytes);
+        ptot += AL256(bp[i].pbytes);
+    }
+    const size_t total_bytes = tab_bytes + start_bytes + wtot + ptot;
+    if (np > 0 || bt->nsc > 0) {
+        int e = cg_malloc(&bt->dmem, total_bytes ? total_bytes : 256);
+        if (e) {
+            bt->dmem = NULL;
+            st = errf(CONVGEMM_ERR_ALLOC, "cannot allocate the batch workspace");
+            goto fail;
+        }
+    }
+    char *base = (char *)bt->dmem;
+    cg_kparams *dtab = (cg_kparams *)base;
+    int *dstart = (int *)(base + tab_bytes);
+    char *wptr = base + tab_bytes + start_bytes;
+    char *pptr = wptr + wtot;
+
+    bt->gl = (glaunch *)calloc((size_t)(ngl > 0 ? ngl : 1), sizeof(glaunch));
+    bt->rep = (repack_t *)calloc((size_t)(nrep > 0 ? nrep : 1), sizeof(repack_t));
+    htab = (cg_kparams *)calloc((size_t)(np + nsplit > 0 ? np + nsplit : 1), sizeof(cg_kparams));
+    hstart = (int *)calloc((size_t)(np + ngl + nsplit + nops > 0 ? np + ngl + nsplit + nops : 1), sizeof(int));
+    if (!bt->gl || !bt->rep || !htab || !hstart) {
+        st = errf(CONVGEMM_ERR_ALLOC, "host allocation failed");
+        goto fail;
+    }
+    /* workspaces */
+    for (int i = 0; i < np; i++) {
+        if (bp[i].repack) {
+            repack_t *r = &bt->rep[bt->nrep++];
+            r->src = bp[i].wsrc;
+            r->dst = wptr;
+            r->d0 = bp[i].d0; r->d1 = bp[i].d1; r->d2 = bp[i].d2; r->perm = bp[i].perm;
+            bp[i].kp.Bm = wptr;
+            wptr += AL256(bp[i].wbytes);
+        }
+        if (bp[i].pbytes) {
+            bp[i].kp.ws = (float *)pptr;
+            pptr += AL256(bp[i].pbytes);
+        }
+        htab[i] = bp[i].kp;
+    }
+    /* implicit-GEMM groups: item ranges padded to multiples of 8 */
+    int dev = 0, ncu = 0;
+    hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 256;
+    int so = 0; /* offset into hstart */
+    for (int i = 0; i < np;) {
+        int j = i;
+        while (j < np && bp[j].cfg == bp[i].cfg && bp[j].kp.op == bp[i].kp.op && bp[j].kp.vec == bp[i].kp.vec) j++;
+        glaunch *g = &bt->gl[bt->ngl++];
+        g->cfg = bp[i].cfg;
+        g->kp0 = bp[i].kp;
+        long long acc = 0;
+        for (int k = i; k < j; k++) {
+            hstart[so + (k - i)] = (int)acc;
+            acc += (bp[k].items + 7) & ~7LL;
+            if (acc > INT_MAX) {
+                st = errf(CONVGEMM_ERR_UNSUPPORTED, "a batch group has more than 2^31 workgroup items");
+                goto fail;
+            }
+        }
+        hstart[so + (j - i)] = (int)acc;
+        g->g.plist = dtab + i;
+        g->g.start = dstart + so;
+        g->g.nprob = j - i;
+        g->g.total = (int)acc;
+        g->g.grid = (int)acc;
+        if (flags & CONVGEMM_BATCH_PERSISTENT) {
+            int occ = cg_igemm_occupancy(bt->elem, g->cfg, &g->kp0);
+            long long cap = (long long)(occ > 0 ? occ : 1) * ncu;
+            cap &= ~7LL;
+            if (cap < 8) cap = 8;
+            if (cap < g->g.grid) g->g.grid = (int)cap;
+        }
+        so += j - i + 1;
+        i = j;
+    }
+    /* split-K reductions: one grouped launch per operation */
+    int nr = 0;
+    for (int i = 0; i < np; i++)
+        if (bp[i].kp.splits > 1) htab[np + nr++] = bp[i].kp;
+    qsort(htab + np, (size_t)nr, sizeof(cg_kparams), cmp_red);
+    for (int i = 0; i < nr;) {
+        int j = i;
+        while (j < nr && htab[np + j].op == htab[np + i].op) j++;
+        rlaunch *r = &bt->rl[bt->nrl++];
+        r->op = htab[np + i].op;
+        long long acc = 0;
+        for (int k = i; k < j; k++) {
+            hstart[so + (k - i)] = (int)acc;
+            acc += cg_reduce_items(&htab[np + k]);
+            if (acc > INT_MAX) {
+                st = errf(CONVGEMM_ERR_UNSUPPORTED, "a batch reduction has more than 2^31 blocks");
+                goto fail;
+            }
+        }
+        hstart[so + (j - i)] = (int)acc;
+        r->g.plist = dtab + np + i;
+        r->g.start = dstart + so;
+        r->g.nprob = j - i;
+        r->g.total = (int)acc;
+        r->g.grid = (int)acc;
+        if ((flags & CONVGEMM_BATCH_PERSISTENT) && r->g.grid > 8 * ncu) r->g.grid = 8 * ncu;
+        so += j - i + 1;
+        i = j;
+    }
+    if (np + nr > 0) {
+        hipError_t e1 = hipMemcpy(dtab, htab, sizeof(cg_kparams) * (size_t)(np + nr), hipMemcpyHostToDevice);
+        hipError_t e2 = hipMemcpy(dstart, hstart, sizeof(int) * (size_t)so, hipMemcpyHostToDevice);
+        if (e1 != hipSuccess || e2 != hipSuccess) {
+            st = cg_hip_fail(e1 != hipSuccess ? e1 : e2, "copy of the batch tables");
+            goto fail;
+        }
+    }
+
+    /* ---- static weights: repack now, once ---- */
+    if ((flags & CONVGEMM_BATCH_STATIC_WEIGHTS) && bt->nrep) {
+        for (int i = 0; i < bt->nrep; i++) {
+            const repack_t *r = &bt->rep[i];
+            int e = cg_launch_permute3(r->src, r->dst, r->d0, r->d1, r->d2, r->perm, cg_esize(bt->elem), NULL);
+            if (e) {
+                st = cg_hip_fail(e, "weight repack");
+                goto fail;
+            }
+        }
+        hipError_t e = hipStreamSynchronize(NULL);
+        if (e != hipSuccess) {
+            st = cg_hip_fail(e, "weight repack");
+            goto fail;
+        }
+    }
+
+    /* ---- graph: capture the run once ---- */
+    if (flags & CONVGEMM_BATCH_GRAPH) {
+        hipStream_t cs;
+        hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            st = cg_hip_fail(e, "capture stream");
+            goto fail;
+        }
+        e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            st = issue(bt, cs, !(flags & CONVGEMM_BATCH_STATIC_WEIGHTS));
+            hipError_t e2 = hipStreamEndCapture(cs, &bt->graph);
+            if (st == CONVGEMM_SUCCESS && e2 != hipSuccess) st = cg_hip_fail(e2, "graph capture");
+            if (st == CONVGEMM_SUCCESS) {
+                e2 = hipGraphInstantiate(&bt->exec, bt->graph, NULL, NULL, 0);
+                if (e2 != hipSuccess) {
+                    bt->exec = NULL;
+                    st = cg_hip_fail(e2, "graph instantiation");
+                }
+            }
+        } else {
+            st = cg_hip_fail(e, "graph capture");
+        }
+        hipStreamDestroy(cs);
+        if (st != CONVGEMM_SUCCESS) goto fail;
+    }
+    free(bp);
+    free(htab);
+    free(hstart);
+    *batch = bt;
+    return cg_set_error(CONVGEMM_SUCCESS, "");
+
+fail:
+    free(bp);
+    free(htab);
+    free(hstart);
+    if (bt && bt->dmem) hipDeviceSynchronize();
+    release(bt);
+    return st;
+}
+
+convgemm_status_t convgemm_batch_run(convgemm_batch_t bt, convgemm_stream_t stream) {
+    if (!bt) return errf(CONVGEMM_ERR_INVALID_VALUE, "batch is NULL");
+    if (bt->exec) {
+        hipError_t e = hipGraphLaunch(bt->exec, (hipStream_t)stream);
+        if (e != hipSuccess) return cg_hip_fail(e, "graph launch");
+        return cg_set_error(CONVGEMM_SUCCESS, "");
+    }
+    convgemm_status_t st = issue(bt, (hipStream_t)stream, !(bt->flags & CONVGEMM_BATCH_STATIC_WEIGHTS));
+    if (st != CONVGEMM_SUCCESS) return st;
+    return cg_set_error(CONVGEMM_SUCCESS, "");
+}
+
+convgemm_status_t convgemm_batch_info(convgemm_batch_t bt, int *groups, int *kernels_per_run, int *graph) {
+    if (!bt) return errf(CONVGEMM_ERR_INVALID_VALUE, "batch is NULL");
+    if (groups) *groups = bt->ngl;
+    if (kernels_per_run)
+        *kernels_per_run = bt->nsc + ((bt->flags & CONVGEMM_BATCH_STATIC_WEIGHTS) ? 0 : bt->nrep) + bt->ngl + bt->nrl;
+    if (graph) *graph = bt->exec != NULL;
+    return cg_set_error(CONVGEMM_SUCCESS, "");
+}
+
+void convgemm_batch_destroy(convgemm_batch_t bt) {
+    if (!bt) return;
+    hipDeviceSynchronize();
+    release(bt);
+}

@@ -40,4 +40,14 @@ convgemm_status_t cg_prepare(const convgemm_desc_t *d, convgemm_dtype_t dtype, c
  * split-K factors that leave >= 2 K tiles per split.  Returns the count. */
 int cg_candidates(int elem, const cg_kparams *kp, int *cfgs, int *splits, int max);
 
+/* Plan (cfg, splits) as a call would: forced plan, planner hook, analytic model. */
+int cg_choose_plan(int elem, const cg_kparams *kp, int *cfg, int *splits);
+/* Weight repack the kernels need for this call (arguments of
+ * cg_launch_permute3); 0 when none. */
+int cg_weight_repack(const convgemm_desc_t *d, int elem, int *d0, int *d1, int *d2, int *perm);
+int cg_esize(int elem);
+/* Device allocation through the library's allocator (honours the test hook). */
+int cg_malloc(void **p, size_t bytes);
+convgemm_status_t cg_hip_fail(int err, const char *what);
+
 #endif

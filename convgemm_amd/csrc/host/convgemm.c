@@ -131,6 +131,9 @@ static convgemm_status_t hip_fail(hipError_t e, const char *what) {
     return set_error(CONVGEMM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
+int cg_malloc(void **p, size_t bytes) { return (int)lib_malloc(p, bytes); }
+convgemm_status_t cg_hip_fail(int e, const char *what) { return hip_fail((hipError_t)e, what); }
+
 /* ------------------------------------------------------------- shapes */
 
 int convgemm_out_size(int in, int pad, int dil, int k, int stride) {
@@ -142,6 +145,7 @@ static int elem_of(convgemm_dtype_t dt) {
     return dt == CONVGEMM_BF16 ? CG_ELEM_BF16 : dt == CONVGEMM_FP8_E4M3 ? CG_ELEM_FP8 : CG_ELEM_F32;
 }
 static int esize_of(int elem) { return elem == CG_ELEM_F32 ? 4 : elem == CG_ELEM_BF16 ? 2 : 1; }
+int cg_esize(int elem) { return esize_of(elem); }
 
 static convgemm_status_t validate(const convgemm_desc_t *d, int *ho, int *wo) {
     if (!d) return set_error(CONVGEMM_ERR_INVALID_VALUE, "descriptor is NULL");
@@ -377,6 +381,10 @@ static int weight_repack(const convgemm_desc_t *d, int elem, int *d0, int *d1, i
     return 0;
 }
 
+int cg_weight_repack(const convgemm_desc_t *d, int elem, int *d0, int *d1, int *d2, int *perm) {
+    return weight_repack(d, elem, d0, d1, d2, perm);
+}
+
 static int vec_ok(int elem, const convgemm_desc_t *d, const void *a, const void *b) {
     int vw = 16 / esize_of(elem);
     if (((uintptr_t)a % 16) || ((uintptr_t)b % 16)) return 0;
@@ -449,6 +457,10 @@ static int choose_plan(int elem, const cg_kparams *kp, int *cfg, int *splits) {
     }
     if (*cfg < 0 || !cg_tilecfg_supports(elem, *cfg, kp->op, kp->vec)) cg_default_plan(elem, kp, cfg, splits);
     return *cfg;
+}
+
+int cg_choose_plan(int elem, const cg_kparams *kp, int *cfg, int *splits) {
+    return choose_plan(elem, kp, cfg, splits);
 }
 
 static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, const void *in,

@@ -11,10 +11,8 @@ namespace cg {
 
 // ---- split-K reduction: one thread per (class, m, n) ----
 template <int OP>
-__global__ void __launch_bounds__(256) splitk_reduce_kernel(const cg_kparams p) {
-    const int cls = blockIdx.y;
+__device__ __attribute__((always_inline)) inline void reduce_elem(const cg_kparams &p, int cls, long long idx) {
     const Ctx x = make_ctx(p, cls);
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = (long long)x.Mrows * p.N;
     if (idx >= total) return;
     const int m = (int)(idx / p.N), n = (int)(idx - (long long)m * p.N);
@@ -22,6 +20,32 @@ __global__ void __launch_bounds__(256) splitk_reduce_kernel(const cg_kparams p) 
     float s = 0.f;
     for (int z = 0; z < p.splits; z++) s += p.ws[(((long long)z * ncls + cls) * p.M + m) * p.N + n];
     store_out<OP>(p, x, m, n, s);
+}
+
+template <int OP>
+__global__ void __launch_bounds__(256) splitk_reduce_kernel(const cg_kparams p) {
+    reduce_elem<OP>(p, blockIdx.y, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Grouped: item = one 256-element block of one class of one problem; the
+// blocks of a class cover M*N (the largest class), as in the single launch.
+template <int OP>
+__global__ void __launch_bounds__(256) splitk_reduce_grouped_kernel(const cg_kparams *__restrict__ plist,
+                                                                    const int *__restrict__ start, int nprob,
+                                                                    int total) {
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        int lo = 0, hi = nprob - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (start[mid] <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const cg_kparams &p = plist[lo];
+        const int local = item - start[lo];
+        const int per_cls = (int)(((long long)p.M * p.N + 255) / 256);
+        const int cls = local / per_cls, blk = local - cls * per_cls;
+        reduce_elem<OP>(p, cls, (long long)blk * 256 + threadIdx.x);
+    }
 }
 
 __global__ void scale_kernel(float *out, long long n, float beta) {
@@ -123,6 +147,55 @@ int cg_fp8_num_cfgs(void);
 int cg_fp8_get_cfg(int cfg, cg_tilecfg *out);
 int cg_fp8_supports(int cfg, int op, int vec);
 int cg_fp8_launch(int cfg, const cg_kparams *kp, void *stream);
+int cg_f32_launch_grouped(int cfg, const cg_kparams *kp0, const cg_group *g, void *stream);
+int cg_bf16_launch_grouped(int cfg, const cg_kparams *kp0, const cg_group *g, void *stream);
+int cg_fp8_launch_grouped(int cfg, const cg_kparams *kp0, const cg_group *g, void *stream);
+int cg_f32_occupancy(int cfg, const cg_kparams *kp0);
+int cg_bf16_occupancy(int cfg, const cg_kparams *kp0);
+int cg_fp8_occupancy(int cfg, const cg_kparams *kp0);
+}
+
+extern "C" int cg_launch_igemm_grouped(int elem, int cfg, const cg_kparams *kp0, const cg_group *g, void *stream) {
+    switch (elem) {
+    case CG_ELEM_F32: return cg_f32_launch_grouped(cfg, kp0, g, stream);
+    case CG_ELEM_BF16: return cg_bf16_launch_grouped(cfg, kp0, g, stream);
+    case CG_ELEM_FP8: return cg_fp8_launch_grouped(cfg, kp0, g, stream);
+    }
+    return (int)hipErrorInvalidValue;
+}
+
+extern "C" int cg_igemm_occupancy(int elem, int cfg, const cg_kparams *kp0) {
+    switch (elem) {
+    case CG_ELEM_F32: return cg_f32_occupancy(cfg, kp0);
+    case CG_ELEM_BF16: return cg_bf16_occupancy(cfg, kp0);
+    case CG_ELEM_FP8: return cg_fp8_occupancy(cfg, kp0);
+    }
+    return 0;
+}
+
+extern "C" long long cg_reduce_items(const cg_kparams *kp) {
+    return (long long)kp->ncls_h * kp->ncls_w * (((long long)kp->M * kp->N + 255) / 256);
+}
+
+extern "C" int cg_launch_splitk_reduce_grouped(int op, const cg_group *g, void *stream) {
+    hipStream_t st = (hipStream_t)stream;
+    if (g->total <= 0) return 0;
+#define CG_RG(OPK)                                                                                               \
+    case OPK:                                                                                                    \
+        hipLaunchKernelGGL(splitk_reduce_grouped_kernel<OPK>, dim3((unsigned)g->grid), dim3(256), 0, st, g->plist, \
+                           g->start, g->nprob, g->total);                                                        \
+        break;
+    switch (op) {
+        CG_RG(CG_FWD_NHWC)
+        CG_RG(CG_FWD_NCHW)
+        CG_RG(CG_WGRAD_NHWC)
+        CG_RG(CG_WGRAD_NCHW)
+        CG_RG(CG_DGRAD_NHWC)
+        CG_RG(CG_DGRAD_NCHW)
+    default: return (int)hipErrorInvalidValue;
+    }
+#undef CG_RG
+    return (int)hipGetLastError();
 }
 
 extern "C" int cg_num_tilecfgs(int elem) {

@@ -36,21 +36,35 @@ extern "C" int cg_f32_get_cfg(int cfg, cg_tilecfg *out) {
 
 extern "C" int cg_f32_supports(int cfg, int op, int vec) { return supports<float>(kCfg, kNoVec, kNum, cfg, op, vec); }
 
-extern "C" int cg_f32_launch(int cfg, const cg_kparams *kp, void *stream) {
+static int launch_any(int cfg, const cg_kparams *kp, const cg_group *g, int *occ, void *stream) {
     hipStream_t st = (hipStream_t)stream;
     switch (cfg) {
-    case 0: return launch_cfg<float, 128, 128, 32, 2, 2, true>(kp, st);
-    case 1: return launch_cfg<float, 128, 64, 32, 2, 2, false>(kp, st);
-    case 2: return launch_cfg<float, 64, 128, 32, 2, 2, false>(kp, st);
-    case 3: return launch_cfg<float, 64, 64, 32, 2, 2, true>(kp, st);
-    case 4: return launch_cfg<float, 256, 64, 32, 4, 1, false>(kp, st);
-    case 5: return launch_cfg<float, 128, 128, 16, 2, 2, false>(kp, st);
-    case 6: return launch_cfg<float, 64, 64, 16, 2, 2, true>(kp, st);
-    case 7: return launch_cfg<float, 64, 256, 32, 1, 4, false>(kp, st);
-    case 8: return launch_cfg<float, 256, 64, 16, 4, 1, false>(kp, st);
-    case 9: return launch_cfg<float, 64, 256, 16, 1, 4, false>(kp, st);
-    case 10: return launch_cfg<float, 256, 128, 16, 2, 2, false>(kp, st);
-    case 11: return launch_cfg<float, 128, 64, 16, 2, 2, false>(kp, st);
+    case 0: return launch_cfg<float, 128, 128, 32, 2, 2, true>(kp, g, occ, st);
+    case 1: return launch_cfg<float, 128, 64, 32, 2, 2, false>(kp, g, occ, st);
+    case 2: return launch_cfg<float, 64, 128, 32, 2, 2, false>(kp, g, occ, st);
+    case 3: return launch_cfg<float, 64, 64, 32, 2, 2, true>(kp, g, occ, st);
+    case 4: return launch_cfg<float, 256, 64, 32, 4, 1, false>(kp, g, occ, st);
+    case 5: return launch_cfg<float, 128, 128, 16, 2, 2, false>(kp, g, occ, st);
+    case 6: return launch_cfg<float, 64, 64, 16, 2, 2, true>(kp, g, occ, st);
+    case 7: return launch_cfg<float, 64, 256, 32, 1, 4, false>(kp, g, occ, st);
+    case 8: return launch_cfg<float, 256, 64, 16, 4, 1, false>(kp, g, occ, st);
+    case 9: return launch_cfg<float, 64, 256, 16, 1, 4, false>(kp, g, occ, st);
+    case 10: return launch_cfg<float, 256, 128, 16, 2, 2, false>(kp, g, occ, st);
+    case 11: return launch_cfg<float, 128, 64, 16, 2, 2, false>(kp, g, occ, st);
     }
     return (int)hipErrorInvalidValue;
+}
+
+extern "C" int cg_f32_launch(int cfg, const cg_kparams *kp, void *stream) {
+    return launch_any(cfg, kp, nullptr, nullptr, stream);
+}
+
+extern "C" int cg_f32_launch_grouped(int cfg, const cg_kparams *kp0, const cg_group *g, void *stream) {
+    return launch_any(cfg, kp0, g, nullptr, stream);
+}
+
+extern "C" int cg_f32_occupancy(int cfg, const cg_kparams *kp0) {
+    int occ = 0;
+    if (launch_any(cfg, kp0, nullptr, &occ, nullptr) != 0) return 0;
+    return occ;
 }

@@ -32,17 +32,31 @@ extern "C" int cg_fp8_get_cfg(int cfg, cg_tilecfg *out) {
 
 extern "C" int cg_fp8_supports(int cfg, int op, int vec) { return supports<uint8_t>(kCfg, kNoVec, kNum, cfg, op, vec); }
 
-extern "C" int cg_fp8_launch(int cfg, const cg_kparams *kp, void *stream) {
+static int launch_any(int cfg, const cg_kparams *kp, const cg_group *g, int *occ, void *stream) {
     hipStream_t st = (hipStream_t)stream;
     switch (cfg) {
-    case 0: return launch_cfg<uint8_t, 128, 128, 64, 2, 2, true>(kp, st);
-    case 1: return launch_cfg<uint8_t, 128, 64, 64, 2, 2, false>(kp, st);
-    case 2: return launch_cfg<uint8_t, 64, 128, 64, 2, 2, false>(kp, st);
-    case 3: return launch_cfg<uint8_t, 64, 64, 64, 2, 2, true>(kp, st);
-    case 4: return launch_cfg<uint8_t, 256, 64, 64, 4, 1, false>(kp, st);
-    case 5: return launch_cfg<uint8_t, 256, 128, 64, 2, 2, false>(kp, st);
-    case 6: return launch_cfg<uint8_t, 128, 128, 128, 2, 2, false>(kp, st);
-    case 7: return launch_cfg<uint8_t, 128, 256, 64, 2, 2, false>(kp, st);
+    case 0: return launch_cfg<uint8_t, 128, 128, 64, 2, 2, true>(kp, g, occ, st);
+    case 1: return launch_cfg<uint8_t, 128, 64, 64, 2, 2, false>(kp, g, occ, st);
+    case 2: return launch_cfg<uint8_t, 64, 128, 64, 2, 2, false>(kp, g, occ, st);
+    case 3: return launch_cfg<uint8_t, 64, 64, 64, 2, 2, true>(kp, g, occ, st);
+    case 4: return launch_cfg<uint8_t, 256, 64, 64, 4, 1, false>(kp, g, occ, st);
+    case 5: return launch_cfg<uint8_t, 256, 128, 64, 2, 2, false>(kp, g, occ, st);
+    case 6: return launch_cfg<uint8_t, 128, 128, 128, 2, 2, false>(kp, g, occ, st);
+    case 7: return launch_cfg<uint8_t, 128, 256, 64, 2, 2, false>(kp, g, occ, st);
     }
     return (int)hipErrorInvalidValue;
+}
+
+extern "C" int cg_fp8_launch(int cfg, const cg_kparams *kp, void *stream) {
+    return launch_any(cfg, kp, nullptr, nullptr, stream);
+}
+
+extern "C" int cg_fp8_launch_grouped(int cfg, const cg_kparams *kp0, const cg_group *g, void *stream) {
+    return launch_any(cfg, kp0, g, nullptr, stream);
+}
+
+extern "C" int cg_fp8_occupancy(int cfg, const cg_kparams *kp0) {
+    int occ = 0;
+    if (launch_any(cfg, kp0, nullptr, &occ, nullptr) != 0) return 0;
+    return occ;
 }

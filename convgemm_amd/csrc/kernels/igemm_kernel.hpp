@@ -71,8 +71,10 @@ __device__ inline int ld_kq(int tid, int i) {
 
 constexpr int FAR = -(1 << 29);  // row coordinate that fails every bounds test
 
+// One workgroup item of problem p: an output tile, split-K slice and (DGRAD)
+// stride class; `lin` is the item's linear index within the problem.
 template <typename T, int BM, int BN, int BK, int WM, int WN, int OP, bool VEC>
-__global__ void __launch_bounds__(NT) igemm_kernel(const cg_kparams p) {
+__device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparams &p, const int lin) {
     typedef typename ET<T>::V V;
     constexpr int VW = ET<T>::VEC;
     constexpr int G = ET<T>::G;
@@ -100,15 +102,10 @@ __global__ void __launch_bounds__(NT) igemm_kernel(const cg_kparams p) {
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
-    // ---- block -> (tile, split, class).  Workgroups are dealt to the 8 XCDs
-    // round-robin; give each XCD a contiguous run of tiles so neighbouring
-    // tiles (shared weight panel, shared input halo) meet in one L2.  Within
-    // the run the N tiles of one M panel are adjacent: the gathered operand
-    // (the large one) is then fetched from HBM once, the small weight panel
-    // stays L2-resident (measured: A re-read ntiles times with M fastest).
-    const int nb = gridDim.x, bid = blockIdx.x;
-    const int xcd = bid & 7, per = nb >> 3, rem = nb & 7;
-    const int lin = xcd * per + min(xcd, rem) + (bid >> 3);
+    // ---- item -> (tile, split, class).  The N tiles of one M panel are
+    // adjacent: the gathered operand (the large one) is then fetched from HBM
+    // once and the small weight panel stays L2-resident (with M fastest the
+    // gathered operand was measured re-read once per N tile).
     const int tiles = p.mtiles * p.ntiles;
     const int tile = lin % tiles, rest = lin / tiles;
     const int split = rest % p.splits, cls = rest / p.splits;
@@ -531,6 +528,46 @@ __global__ void __launch_bounds__(NT) igemm_kernel(const cg_kparams p) {
             }
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8);
+// give each XCD a contiguous run of the item space so neighbouring tiles
+// (shared weight panel, shared input halo) meet in one L2.
+__device__ inline int xcd_remap(int bid, int nb) {
+    const int xcd = bid & 7, per = nb >> 3, rem = nb & 7;
+    return xcd * per + min(xcd, rem) + (bid >> 3);
+}
+
+template <typename T, int BM, int BN, int BK, int WM, int WN, int OP, bool VEC>
+__global__ void __launch_bounds__(NT) igemm_kernel(const cg_kparams p) {
+    igemm_tile<T, BM, BN, BK, WM, WN, OP, VEC>(p, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// Many independent problems of one configuration in one launch: item i
+// belongs to the problem j with start[j] <= i < start[j+1] (nprob + 1
+// entries).  Every start[j] is a multiple of 8 and a problem's range is its
+// workgroup count rounded up to 8, so items 8k..8k+7 of a problem run on the
+// 8 different XCDs and the XCD remap can be applied per problem: each XCD
+// gets a contiguous run of every problem's tiles, and every problem is
+// spread over all XCDs whatever the mix of problem costs.  Workgroup b runs
+// items b, b + gridDim.x, ...; gridDim.x is the item count, or a multiple of
+// 8 (persistent workgroups) so that all items of a workgroup keep its XCD.
+template <typename T, int BM, int BN, int BK, int WM, int WN, int OP, bool VEC>
+__global__ void __launch_bounds__(NT) igemm_grouped_kernel(const cg_kparams *__restrict__ plist,
+                                                           const int *__restrict__ start, int nprob, int total) {
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        int lo = 0, hi = nprob - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (start[mid] <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const cg_kparams &p = plist[lo];
+        const int lin = xcd_remap(item - start[lo], start[lo + 1] - start[lo]);
+        const int count = p.mtiles * p.ntiles * p.splits * p.ncls_h * p.ncls_w;
+        __syncthreads();  // LDS reads of the previous item are complete
+        if (lin < count) igemm_tile<T, BM, BN, BK, WM, WN, OP, VEC>(p, lin);
+    }
+}
+
 // ---- instantiation helpers for the per-type launch files ----
 
 // FP32 and BF16 serve all six operations; FP8 serves the forward pass.
@@ -543,8 +580,21 @@ template <typename T> constexpr bool vec_relevant(int op) {
            op == CG_DGRAD_NCHW;
 }
 
+// g == nullptr: one problem (kp), one workgroup per item.  g != nullptr: the
+// grouped kernel over g's problems (kp only selects op and vec).  occ !=
+// nullptr: report resident workgroups per CU instead of launching.
 template <typename T, int BM, int BN, int BK, int WM, int WN, int OP, bool VEC>
-inline int launch_igemm(const cg_kparams *kp, hipStream_t st) {
+inline int launch_igemm(const cg_kparams *kp, const cg_group *g, int *occ, hipStream_t st) {
+    if (occ) {
+        return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            occ, reinterpret_cast<const void *>(&igemm_grouped_kernel<T, BM, BN, BK, WM, WN, OP, VEC>), NT, 0);
+    }
+    if (g) {
+        if (g->total <= 0) return 0;
+        hipLaunchKernelGGL((igemm_grouped_kernel<T, BM, BN, BK, WM, WN, OP, VEC>), dim3((unsigned)g->grid), dim3(NT),
+                           0, st, g->plist, g->start, g->nprob, g->total);
+        return (int)hipGetLastError();
+    }
     const long long tiles = (long long)kp->mtiles * kp->ntiles;
     const long long nb = tiles * kp->splits * kp->ncls_h * kp->ncls_w;
     hipLaunchKernelGGL((igemm_kernel<T, BM, BN, BK, WM, WN, OP, VEC>), dim3((unsigned)nb), dim3(NT), 0, st, *kp);
@@ -553,27 +603,27 @@ inline int launch_igemm(const cg_kparams *kp, hipStream_t st) {
 
 // NV: this configuration also has element-load (non-VEC) instantiations.
 template <typename T, int BM, int BN, int BK, int WM, int WN, bool NV, int OP>
-inline int launch_op(const cg_kparams *kp, hipStream_t st) {
+inline int launch_op(const cg_kparams *kp, const cg_group *g, int *occ, hipStream_t st) {
     if constexpr (!op_enabled<T>(OP)) {
         return (int)hipErrorInvalidValue;
     } else if constexpr (!vec_relevant<T>(OP)) {
-        return launch_igemm<T, BM, BN, BK, WM, WN, OP, true>(kp, st);
+        return launch_igemm<T, BM, BN, BK, WM, WN, OP, true>(kp, g, occ, st);
     } else {
-        if (kp->vec) return launch_igemm<T, BM, BN, BK, WM, WN, OP, true>(kp, st);
-        if constexpr (NV) return launch_igemm<T, BM, BN, BK, WM, WN, OP, false>(kp, st);
+        if (kp->vec) return launch_igemm<T, BM, BN, BK, WM, WN, OP, true>(kp, g, occ, st);
+        if constexpr (NV) return launch_igemm<T, BM, BN, BK, WM, WN, OP, false>(kp, g, occ, st);
         return (int)hipErrorInvalidValue;
     }
 }
 
 template <typename T, int BM, int BN, int BK, int WM, int WN, bool NV>
-inline int launch_cfg(const cg_kparams *kp, hipStream_t st) {
+inline int launch_cfg(const cg_kparams *kp, const cg_group *g, int *occ, hipStream_t st) {
     switch (kp->op) {
-    case CG_FWD_NHWC: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_FWD_NHWC>(kp, st);
-    case CG_FWD_NCHW: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_FWD_NCHW>(kp, st);
-    case CG_WGRAD_NHWC: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_WGRAD_NHWC>(kp, st);
-    case CG_WGRAD_NCHW: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_WGRAD_NCHW>(kp, st);
-    case CG_DGRAD_NHWC: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_DGRAD_NHWC>(kp, st);
-    case CG_DGRAD_NCHW: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_DGRAD_NCHW>(kp, st);
+    case CG_FWD_NHWC: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_FWD_NHWC>(kp, g, occ, st);
+    case CG_FWD_NCHW: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_FWD_NCHW>(kp, g, occ, st);
+    case CG_WGRAD_NHWC: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_WGRAD_NHWC>(kp, g, occ, st);
+    case CG_WGRAD_NCHW: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_WGRAD_NCHW>(kp, g, occ, st);
+    case CG_DGRAD_NHWC: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_DGRAD_NHWC>(kp, g, occ, st);
+    case CG_DGRAD_NCHW: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_DGRAD_NCHW>(kp, g, occ, st);
     }
     return (int)hipErrorInvalidValue;
 }

@@ -1,0 +1,219 @@
+"""Batched grouped launches and hipGraph replay (include/convgemm_batch.h).
+
+A batch runs the same kernels with the same plans as one convgemm_conv_ex
+call per problem, so every output must be BIT-identical to the
+per-problem call: the grouped kernel only changes which workgroup computes
+which tile, never the arithmetic or its order.  Problems mix all six
+operations, epilogues, alpha/beta, split-K, stride classes, BF16 and FP8.
+Graph batches are replayed several times with eager work (a fill, a
+conv_ex call) and synchronizations between replays, and every replay is
+checked.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+import convgemm_amd as cg  # noqa: E402
+from tests.gpu_util import DEV  # noqa: E402
+
+FLAG_SETS = [0, cg.BATCH_GRAPH, cg.BATCH_PERSISTENT, cg.BATCH_GRAPH | cg.BATCH_PERSISTENT,
+             cg.BATCH_GRAPH | cg.BATCH_STATIC_WEIGHTS]
+
+
+def out_hw(h, pad, dil, k, s):
+    return (h + 2 * pad - dil * (k - 1) - 1) // s + 1
+
+
+def sizes(d):
+    ho, wo = out_hw(d.h, d.vpad, d.vdil, d.kh, d.vstride), out_hw(d.w, d.hpad, d.hdil, d.kw, d.hstride)
+    nx, nw, ny = d.b * d.h * d.w * d.c, d.c * d.kh * d.kw * d.kn, d.b * ho * wo * d.kn
+    if d.op == cg.FWD:
+        return nw, nx, ny  # in, x, out
+    if d.op == cg.WGRAD:
+        return ny, nx, nw
+    return nw, ny, nx
+
+
+def operand(n, dtype, gen):
+    r = torch.randn(n, generator=gen, device="cpu").to(DEV)
+    if dtype == cg.BF16:
+        return r.to(torch.bfloat16).view(torch.int16)
+    if dtype == cg.FP8_E4M3:
+        return r.to(torch.float8_e4m3fn).view(torch.uint8)
+    return r
+
+
+class Case:
+    """One problem with its operands, a pristine output and the per-call reference."""
+
+    def __init__(self, desc, dtype, seed, alpha=1.0, beta=0.0, epi=None):
+        g = torch.Generator().manual_seed(seed)
+        ni, nx, no = sizes(desc)
+        self.desc, self.dtype, self.alpha, self.beta = desc, dtype, alpha, beta
+        self.in_, self.x = operand(ni, dtype, g), operand(nx, dtype, g)
+        self.out0 = torch.randn(no, generator=g).to(DEV)
+        self.out = self.out0.clone()
+        self.scales = (0.5, 0.25) if dtype == cg.FP8_E4M3 else None
+        self.epi = {}
+        if epi and desc.op == cg.FWD:
+            kn = desc.kn
+            for k in epi:
+                if k == "relu":
+                    self.epi["relu"] = True
+                elif k == "inv_std":
+                    self.epi[k] = (torch.rand(kn, generator=g) + 0.5).to(DEV)
+                else:
+                    self.epi[k] = (0.1 * torch.randn(kn, generator=g)).to(DEV)
+
+    def problem(self):
+        return cg.problem(self.desc, self.in_, self.x, self.out, alpha=self.alpha, beta=self.beta,
+                          scales=self.scales, **self.epi)
+
+    def reference(self):
+        ref = self.out0.clone()
+        cg.conv_ex(self.desc, self.in_, self.x, ref, alpha=self.alpha, beta=self.beta, dtype=self.dtype,
+                   scales=self.scales, **self.epi)
+        torch.cuda.synchronize()
+        return ref
+
+    def reset(self):
+        self.out.copy_(self.out0)
+
+
+def D(op, layout, b, h, w, c, kn, k, pad=None, s=1, dil=1):
+    pad = k // 2 if pad is None else pad
+    return cg.make_desc(op, layout, b, h, w, c, kn, k, k, pad, pad, s, s, dil, dil)
+
+
+BN_RELU = ("bias", "mean", "inv_std", "gamma", "bnbeta", "relu")
+
+
+def mixed_f32():
+    F, W, G = cg.FWD, cg.WGRAD, cg.DGRAD
+    return [
+        Case(D(F, cg.NHWC, 2, 14, 14, 32, 48, 3), cg.F32, 1, epi=BN_RELU),
+        Case(D(F, cg.NCHW, 1, 9, 11, 20, 36, 3, s=2), cg.F32, 2, epi=("bias", "relu")),
+        Case(D(F, cg.NHWC, 4, 7, 7, 64, 64, 1), cg.F32, 3, alpha=0.5, beta=2.0),
+        Case(D(F, cg.NHWC, 1, 28, 28, 16, 24, 5, s=2), cg.F32, 4),
+        Case(D(W, cg.NHWC, 2, 10, 10, 24, 40, 3), cg.F32, 5),
+        Case(D(W, cg.NCHW, 2, 12, 9, 16, 20, 3, s=2), cg.F32, 6, beta=1.0),
+        Case(D(G, cg.NHWC, 2, 13, 13, 24, 32, 3, s=2), cg.F32, 7, beta=1.0),
+        Case(D(G, cg.NCHW, 1, 10, 12, 18, 22, 3, dil=2), cg.F32, 8, beta=1.0),
+        Case(D(F, cg.NCHW, 3, 16, 16, 40, 24, 1), cg.F32, 9, epi=("mean", "inv_std", "gamma", "bnbeta")),
+        Case(D(F, cg.NHWC, 2, 8, 8, 16, 16, 3), cg.F32, 10, alpha=0.0, beta=3.0),  # beta scaling only
+        Case(D(F, cg.NHWC, 0, 8, 8, 16, 16, 3), cg.F32, 11),  # empty batch: dropped
+    ]
+
+
+def check(cases, refs, tag):
+    for i, (c, r) in enumerate(zip(cases, refs)):
+        got, exp = c.out.cpu().numpy(), r.cpu().numpy()
+        assert np.array_equal(got, exp), (tag, i, float(np.max(np.abs(got - exp))))
+
+
+@pytest.mark.parametrize("flags", FLAG_SETS)
+def test_mixed_f32_bit_identical(flags):
+    cases = mixed_f32()
+    refs = [c.reference() for c in cases]
+    b = cg.Batch([c.problem() for c in cases], cg.F32, flags)
+    groups, kernels, graph = b.info()
+    assert groups >= 1 and kernels >= groups and graph == bool(flags & cg.BATCH_GRAPH)
+    for rep in range(3 if flags & cg.BATCH_GRAPH else 1):
+        for c in cases:
+            c.reset()  # eager fills between replays
+        torch.cuda.synchronize()
+        b.run()
+        torch.cuda.synchronize()
+        check(cases, refs, f"flags={flags} replay={rep}")
+        # unrelated eager library call between replays
+        cases[0].reference()
+    b.close()
+
+
+@pytest.mark.parametrize("cfg,splits", [(3, 3), (0, 4), (6, 2)])
+def test_split_k_groups(cfg, splits):
+    cg.force_config(cfg, splits)
+    try:
+        cases = [c for c in mixed_f32() if c.desc.b > 0 and c.alpha != 0.0]
+        refs = [c.reference() for c in cases]
+        b = cg.Batch([c.problem() for c in cases], cg.F32, cg.BATCH_GRAPH)
+    finally:
+        cg.force_config(-1, 0)
+    for rep in range(2):
+        for c in cases:
+            c.reset()
+        b.run()
+        torch.cuda.synchronize()
+        check(cases, refs, f"cfg={cfg} splits={splits} replay={rep}")
+    b.close()
+
+
+def uneven(n, dtype, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        k = int(rng.choice([1, 3, 5]))
+        s = int(rng.choice([1, 2]))
+        h = int(rng.choice([7, 14, 28]))
+        c = int(rng.choice(range(8, 129, 8)))
+        kn = int(rng.choice(range(8, 129, 8)))
+        lay = int(rng.integers(2))
+        out.append(Case(D(cg.FWD, lay, int(rng.choice([1, 2, 4])), h, h, c, kn, k, s=s), dtype, 1000 + i,
+                        epi=BN_RELU if i % 3 == 0 else None))
+    return out
+
+
+@pytest.mark.parametrize("dtype", [cg.F32, cg.BF16, cg.FP8_E4M3])
+@pytest.mark.parametrize("flags", [cg.BATCH_GRAPH, cg.BATCH_GRAPH | cg.BATCH_PERSISTENT])
+def test_uneven_forward_batch(dtype, flags):
+    cases = uneven(40, dtype, 77 + dtype)
+    refs = [c.reference() for c in cases]
+    b = cg.Batch([c.problem() for c in cases], dtype, flags)
+    for rep in range(2):
+        for c in cases:
+            c.reset()
+        torch.cuda.synchronize()
+        b.run(torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        check(cases, refs, f"dtype={dtype} flags={flags} replay={rep}")
+    b.close()
+
+
+def test_bf16_backward_batch():
+    F, W, G = cg.FWD, cg.WGRAD, cg.DGRAD
+    cases = [Case(D(W, cg.NHWC, 2, 10, 10, 32, 48, 3), cg.BF16, 21),
+             Case(D(G, cg.NHWC, 2, 11, 11, 40, 24, 3, s=2), cg.BF16, 22, beta=1.0),
+             Case(D(W, cg.NCHW, 1, 9, 9, 24, 16, 3), cg.BF16, 23),
+             Case(D(G, cg.NCHW, 2, 8, 10, 16, 32, 3), cg.BF16, 24, beta=1.0),
+             Case(D(F, cg.NCHW, 2, 8, 8, 16, 32, 3), cg.BF16, 25, epi=BN_RELU)]
+    refs = [c.reference() for c in cases]
+    b = cg.Batch([c.problem() for c in cases], cg.BF16, cg.BATCH_GRAPH | cg.BATCH_STATIC_WEIGHTS)
+    for c in cases:
+        c.reset()
+    b.run()
+    torch.cuda.synchronize()
+    check(cases, refs, "bf16 backward")
+    b.close()
+
+
+def test_batch_errors():
+    c = Case(D(cg.FWD, cg.NHWC, 1, 8, 8, 8, 8, 3), cg.F32, 31)
+    pr = c.problem()
+    pr.out = None
+    with pytest.raises(cg.ConvGemmError):
+        cg.Batch([pr], cg.F32, 0)
+    pr = c.problem()
+    pr.desc.vstride = 0
+    with pytest.raises(cg.ConvGemmError):
+        cg.Batch([pr], cg.F32, 0)
+    with pytest.raises(cg.ConvGemmError):
+        cg.Batch([c.problem()], cg.F32, 64)  # unknown flag
+    with pytest.raises(cg.ConvGemmError):  # FP8 is forward-only
+        cg.Batch([Case(D(cg.WGRAD, cg.NHWC, 1, 8, 8, 16, 16, 3), cg.FP8_E4M3, 32).problem()], cg.FP8_E4M3, 0)
+    empty = cg.Batch([], cg.F32, cg.BATCH_GRAPH)
+    empty.run()
+    torch.cuda.synchronize()
+    assert empty.info()[0] == 0
+    empty.close()
