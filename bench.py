@@ -155,6 +155,62 @@ def precision_sweep(torch, cg, stream, tune=True, reps=10, batch=64):
     return res
 
 
+def batch_sweep(torch, cg, stream, n=256, reps=10, seed=5):
+    """Many uneven BF16 forward layers (NHWC; batch {1,4,16}, H=W {7..56},
+    Cin/Cout 32..1024, k {1,3,5}, s {1,2}): one conv_ex call per problem vs
+    one batch (grouped launches captured in a hipGraph), plans from the
+    learned cost model shipped in convgemm_amd/data (trained on autotuner
+    timings).  Returns aggregate TFLOP/s of each way of launching."""
+    from convgemm_amd import tuning
+    model = os.path.join(ROOT, "convgemm_amd", "data", "mi355x_bf16_fwd_nhwc.mlp")
+    cg.costmodel_load(model)
+    cg.set_planner(cg.PLAN_MODEL)
+    try:
+        probs = tuning.random_problems(n, seed)
+        g = torch.Generator(device="cuda").manual_seed(0xC0FFEE + 5000)
+        ops, flops = [], 0.0
+        for p in probs:
+            d = tuning.desc_of(p)
+            ho = out_hw(p["h"], p["pad"], p["k"], p["s"])
+            x = torch.randn(p["b"] * p["h"] * p["h"] * p["c"], device="cuda", generator=g)
+            w = torch.randn(p["c"] * p["k"] * p["k"] * p["kn"], device="cuda", generator=g) * 0.05
+            y = torch.empty(p["b"] * ho * ho * p["kn"], device="cuda")
+            ops.append((d, w.to(torch.bfloat16).view(torch.int16), x.to(torch.bfloat16).view(torch.int16), y))
+            flops += 2.0 * p["b"] * ho * ho * p["c"] * p["k"] * p["k"] * p["kn"]
+
+        def timed(fn):
+            for _ in range(2):
+                fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) / reps
+
+        def sequential():
+            for d, w, x, y in ops:
+                cg.conv_ex(d, w, x, y, dtype=cg.BF16, stream=stream)
+
+        res = {"problems": n, "dtype": "bf16", "gflop": round(flops / 1e9, 3), "planner": "learned cost model"}
+        ms = timed(sequential)
+        res["sequential_calls"] = {"ms": round(ms, 4), "tflops": round(flops / (ms * 1e-3) / 1e12, 2)}
+        for key, fl in (("batch_graph", cg.BATCH_GRAPH),
+                        ("batch_graph_persistent", cg.BATCH_GRAPH | cg.BATCH_PERSISTENT)):
+            b = cg.Batch([cg.problem(d, w, x, y) for d, w, x, y in ops], cg.BF16, fl)
+            ms = timed(lambda: b.run(stream))
+            gr, kern, _ = b.info()
+            res[key] = {"ms": round(ms, 4), "tflops": round(flops / (ms * 1e-3) / 1e12, 2), "groups": gr,
+                        "kernels_per_run": kern}
+            b.close()
+        res["speedup_graph_vs_sequential"] = round(res["sequential_calls"]["ms"] / res["batch_graph"]["ms"], 2)
+        return res
+    finally:
+        cg.set_planner(cg.PLAN_TUNED)
+
+
 def cpu_baseline(budget_s=12.0):
     """Time the oracle's FP32 restatement of the reference's implicit GEMM
     (oracle/convgemm_port.c) on the host cores, on the same layer sweep at
@@ -222,6 +278,7 @@ def main(argv=None):
     ap.add_argument("--no-error", action="store_true", help="skip the FP64 error check")
     ap.add_argument("--no-tune", action="store_true", help="use the analytic planner instead of autotuning at setup")
     ap.add_argument("--no-precision", action="store_true", help="skip the BF16/FP8 precision sweep")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-layer sweep")
     ap.add_argument("--tune-db", default="", help="tuning database: loaded if present (layers found there are "
                     "not re-tuned), written back after tuning")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -351,6 +408,8 @@ def main(argv=None):
     }
     if not a.no_precision:
         out["precision_sweep"] = precision_sweep(torch, cg, stream, tune=not a.no_tune)
+    if not a.no_batch:
+        out["batch_sweep"] = batch_sweep(torch, cg, stream)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
