@@ -29,6 +29,10 @@ ARCH = "gfx950"
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in VGPRs: with AGPR accumulators the loop-carried
+    # values were copied AGPR<->VGPR every K step (measured 8x more VALU than
+    # MFMA instructions in the BF16 kernels) and occupancy dropped
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
     "-Wno-unused-result",
 ]
