@@ -47,6 +47,10 @@ EXPORTED_SYMBOLS = (
     "convgemm_costmodel_load", "convgemm_costmodel_predict",
     # convgemm_batch.h
     "convgemm_batch_create", "convgemm_batch_run", "convgemm_batch_info", "convgemm_batch_destroy",
+    # convgemm_explicit.h
+    "im2row_nhwc", "row2im_nhwc", "im2col_nchw", "col2im_nchw", "transpose_nchw", "convgemm_gemm",
+    "convgemm_nchw_to_nhwc", "convgemm_nhwc_to_nchw", "convgemm_weights_nhwc_to_nchw",
+    "convgemm_weights_nchw_to_nhwc",
 )
 
 PLAN_ANALYTIC, PLAN_TUNED, PLAN_LEARNED, PLAN_MODEL = 0, 1, 2, 3
@@ -164,6 +168,23 @@ def lib() -> ctypes.CDLL:
     L.convgemm_batch_info.restype = ci
     L.convgemm_batch_destroy.argtypes = [vp]
     L.convgemm_batch_destroy.restype = None
+    L.im2row_nhwc.argtypes = [vp, ci, vp] + [ci] * 14
+    L.im2row_nhwc.restype = None
+    L.im2col_nchw.argtypes = L.im2row_nhwc.argtypes
+    L.im2col_nchw.restype = None
+    L.row2im_nhwc.argtypes = [ci, ci, vp, ci, vp] + [ci] * 14
+    L.row2im_nhwc.restype = None
+    L.col2im_nchw.argtypes = L.row2im_nhwc.argtypes
+    L.col2im_nchw.restype = None
+    L.transpose_nchw.argtypes = [ci, ci, vp, ci, cf, vp, ci, ci, ci, ci, ci]
+    L.transpose_nchw.restype = None
+    ch = ctypes.c_char
+    L.convgemm_gemm.argtypes = [ch, ch, ch, ch, ch, ci, ci, ci, cf, vp, ci, vp, ci, cf, vp, ci, ci, vp, vp]
+    L.convgemm_gemm.restype = ci
+    for f in ("convgemm_nchw_to_nhwc", "convgemm_nhwc_to_nchw", "convgemm_weights_nhwc_to_nchw",
+              "convgemm_weights_nchw_to_nhwc"):
+        getattr(L, f).argtypes = [vp, vp, ci, ci, ci, ci, ci, vp]
+        getattr(L, f).restype = ci
     L.cg_force_config.argtypes = [ci, ci]
     L.cg_force_config.restype = None
     L.cg_last_plan.argtypes = [ctypes.POINTER(ci)] * 3
@@ -481,3 +502,83 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ---- explicit path (include/convgemm_explicit.h) ----
+
+def im2row_nhwc(rows, ld, in_, batches, height, width, channels, oheight, owidth, kheight, kwidth, vpadding,
+                hpadding, vstride, hstride, vdilation, hdilation):
+    """rows[p*ld + r] = in[b][ix][iy][c] for in-bounds taps (src/im2row_nhwc.c:147-204)."""
+    L = lib()
+    L.im2row_nhwc(_ptr(rows), ld, _ptr(in_), batches, height, width, channels, oheight, owidth, kheight, kwidth,
+                  vpadding, hpadding, vstride, hstride, vdilation, hdilation)
+    _check(L)
+
+
+def row2im_nhwc(m, n, rows, ld, out, batches, height, width, channels, oheight, owidth, kheight, kwidth, vpadding,
+                hpadding, vstride, hstride, vdilation, hdilation):
+    """out += row2im(rows) (src/im2row_nhwc.c:206-229)."""
+    L = lib()
+    L.row2im_nhwc(m, n, _ptr(rows), ld, _ptr(out), batches, height, width, channels, oheight, owidth, kheight,
+                  kwidth, vpadding, hpadding, vstride, hstride, vdilation, hdilation)
+    _check(L)
+
+
+def im2col_nchw(cols, ld, in_, batches, channels, height, width, oheight, owidth, kheight, kwidth, vpadding,
+                hpadding, vstride, hstride, vdilation, hdilation):
+    """cols[r*ld + p] = in[b][c][ix][iy] for in-bounds taps (src/im2col_nchw.c:146-204)."""
+    L = lib()
+    L.im2col_nchw(_ptr(cols), ld, _ptr(in_), batches, channels, height, width, oheight, owidth, kheight, kwidth,
+                  vpadding, hpadding, vstride, hstride, vdilation, hdilation)
+    _check(L)
+
+
+def col2im_nchw(m, n, cols, ld, out, batches, channels, height, width, oheight, owidth, kheight, kwidth, vpadding,
+                hpadding, vstride, hstride, vdilation, hdilation):
+    """out += col2im(cols) (src/im2col_nchw.c:337-359)."""
+    L = lib()
+    L.col2im_nchw(m, n, _ptr(cols), ld, _ptr(out), batches, channels, height, width, oheight, owidth, kheight,
+                  kwidth, vpadding, hpadding, vstride, hstride, vdilation, hdilation)
+    _check(L)
+
+
+def transpose_nchw(rows, cols, in_, ld, beta, out, kn, ho, wo, start_row, start_col):
+    """[kn-block][b][ho][wo] -> [b][kn][ho][wo] with beta (src/im2col_nchw.c:206-233)."""
+    L = lib()
+    L.transpose_nchw(rows, cols, _ptr(in_), ld, float(beta), _ptr(out), kn, ho, wo, start_row, start_col)
+    _check(L)
+
+
+def gemm(orderA, orderB, orderC, transA, transB, m, n, k, alpha, A, ldA, B, ldB, beta, C, ldC, dtype=F32,
+         scales=None, stream=None):
+    """C = alpha*op(A)*op(B) + beta*C on the implicit-GEMM kernels (convgemm_gemm)."""
+    L = lib()
+    enc = lambda ch: ch.encode() if isinstance(ch, str) else ch  # noqa: E731
+    st = L.convgemm_gemm(enc(orderA), enc(orderB), enc(orderC), enc(transA), enc(transB), m, n, k, float(alpha),
+                         _ptr(A), ldA, _ptr(B), ldB, float(beta), _ptr(C), ldC, int(dtype), _scales_ptr(scales),
+                         _stream_handle(stream))
+    if st != 0:
+        raise ConvGemmError(st, L.convgemm_last_error_message().decode())
+
+
+def _conv_layout(fn, src, dst, a, b, c, d, esize, stream):
+    L = lib()
+    st = getattr(L, fn)(_ptr(src), _ptr(dst), a, b, c, d, esize, _stream_handle(stream))
+    if st != 0:
+        raise ConvGemmError(st, L.convgemm_last_error_message().decode())
+
+
+def nchw_to_nhwc(src, dst, b, c, h, w, esize=4, stream=None):
+    _conv_layout("convgemm_nchw_to_nhwc", src, dst, b, c, h, w, esize, stream)
+
+
+def nhwc_to_nchw(src, dst, b, h, w, c, esize=4, stream=None):
+    _conv_layout("convgemm_nhwc_to_nchw", src, dst, b, h, w, c, esize, stream)
+
+
+def weights_nhwc_to_nchw(src, dst, c, kh, kw, kn, esize=4, stream=None):
+    _conv_layout("convgemm_weights_nhwc_to_nchw", src, dst, c, kh, kw, kn, esize, stream)
+
+
+def weights_nchw_to_nhwc(src, dst, kn, c, kh, kw, esize=4, stream=None):
+    _conv_layout("convgemm_weights_nchw_to_nhwc", src, dst, kn, c, kh, kw, esize, stream)

@@ -100,6 +100,25 @@ int cg_launch_splitk_reduce_grouped(int op, const cg_group *g, void *stream);
 long long cg_reduce_items(const cg_kparams *kp);
 /* Resident workgroups per CU of a configuration (occupancy), or 0. */
 int cg_igemm_occupancy(int elem, int cfg, const cg_kparams *kp0);
+
+/* ---- explicit transforms (FP32) and copies ---- */
+typedef struct {
+    int b, h, w, c, ho, wo, kh, kw, vp, hp, vs, hs, vd, hd;
+} cg_tshape;
+/* nchw = 0: rows[p*ld + r] = X[b][ix][iy][c] (im2row);  nchw = 1:
+ * cols[r*ld + p] = X[b][c][ix][iy] (im2col); only in-bounds taps written.
+ * p = (b*ho + x)*wo + y, r = (c*kh + kx)*kw + ky. */
+int cg_launch_im2row(const cg_tshape *s, const float *x, float *out, long long ld, int nchw, void *stream);
+/* X += row2im(rows) (nchw = 0) or col2im(cols) (nchw = 1). */
+int cg_launch_row2im(const cg_tshape *s, const float *in, long long ld, float *x, int nchw, void *stream);
+int cg_launch_transpose_nchw(int rows, int cols, const float *in, long long ld, float beta, float *out, int kn,
+                             int ho, int wo, int start_row, int start_col, void *stream);
+/* dst[r*ldd + c] = src[r*lds + c] for r < rows, c < cols; esize 4, 2 or 1. */
+int cg_launch_copy2d(const void *src, long long lds, void *dst, long long ldd, long long rows, long long cols,
+                     int esize, void *stream);
+/* dst[c*ldd + r] = src[r*lds + c] for r < rows (< 2^21), c < cols. */
+int cg_launch_transpose2d(const void *src, long long lds, void *dst, long long ldd, int rows, int cols, int esize,
+                          void *stream);
 /* out[i] = beta*out[i] (0 if beta == 0), n elements. */
 int cg_launch_scale(float *out, long long n, float beta, void *stream);
 /* dst = src with its three dimensions [d0][d1][d2] permuted:

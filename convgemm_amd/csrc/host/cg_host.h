@@ -49,5 +49,7 @@ int cg_esize(int elem);
 /* Device allocation through the library's allocator (honours the test hook). */
 int cg_malloc(void **p, size_t bytes);
 convgemm_status_t cg_hip_fail(int err, const char *what);
+/* The library stream of the drop-in calls (hipStream_t). */
+void *cg_get_stream(void);
 
 #endif

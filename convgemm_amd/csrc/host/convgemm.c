@@ -132,6 +132,7 @@ static convgemm_status_t hip_fail(hipError_t e, const char *what) {
 }
 
 int cg_malloc(void **p, size_t bytes) { return (int)lib_malloc(p, bytes); }
+void *cg_get_stream(void) { return (void *)g_stream; }
 convgemm_status_t cg_hip_fail(int e, const char *what) { return hip_fail((hipError_t)e, what); }
 
 /* ------------------------------------------------------------- shapes */

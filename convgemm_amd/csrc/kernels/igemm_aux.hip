@@ -289,6 +289,19 @@ extern "C" int cg_launch_permute3(const void *src, void *dst, int d0, int d1, in
     return (int)hipErrorInvalidValue;
 }
 
+extern "C" int cg_launch_transpose2d(const void *src, long long lds, void *dst, long long ldd, int rows, int cols,
+                                     int esize, void *stream) {
+    if (rows <= 0 || cols <= 0) return 0;
+    if ((rows + 31) / 32 > 65535) return (int)hipErrorInvalidValue;
+    hipStream_t st = (hipStream_t)stream;
+    switch (esize) {
+    case 4: return launch_transpose<float>(src, dst, rows, cols, 1, 0, lds, 0, ldd, st);
+    case 2: return launch_transpose<uint16_t>(src, dst, rows, cols, 1, 0, lds, 0, ldd, st);
+    case 1: return launch_transpose<uint8_t>(src, dst, rows, cols, 1, 0, lds, 0, ldd, st);
+    }
+    return (int)hipErrorInvalidValue;
+}
+
 // FP32 -> BF16 (scale = 1) or FP8 e4m3fn with a per-tensor amax scale.
 // `scratch` holds 8 device bytes (amax bits + scale); `scale_dev` receives the scale.
 extern "C" int cg_launch_quantize(const float *src, void *dst, long long n, int elem, void *scratch, void *stream) {

@@ -251,6 +251,57 @@ def test_pack_buffers_and_caller_workspace():
         cg.force_config(-1, 0)
 
 
+def test_dropin_call_sequence():
+    """The inferred caller's sequence through bare ctypes declarations (no wrappers):
+    alloc_pack_buffs -> sconvGemmNHWC 'N' / 'T' -> sconvGemmNHWC_back -> the NCHW
+    three -> convgemm_free_pack_buffs, checked against the oracle."""
+    import ctypes
+    lib = ctypes.CDLL(cg.LIB_PATH)
+    vp, ci = ctypes.c_void_p, ctypes.c_int
+    lib.alloc_pack_buffs.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp)]
+    lib.alloc_pack_buffs.restype = ci
+    lib.sconvGemmNHWC.argtypes = [ctypes.c_char] + [ci] * 13 + [vp] * 8 + [ctypes.c_bool, vp, vp]
+    lib.sconvGemmNHWC.restype = None
+    lib.sconvGemmNCHW.argtypes = lib.sconvGemmNHWC.argtypes
+    lib.sconvGemmNCHW.restype = None
+    lib.sconvGemmNHWC_back.argtypes = [ci] * 13 + [vp] * 5
+    lib.sconvGemmNHWC_back.restype = None
+    lib.sconvGemmNCHW_back.argtypes = lib.sconvGemmNHWC_back.argtypes
+    lib.sconvGemmNCHW_back.restype = None
+    lib.convgemm_free_pack_buffs.argtypes = [vp, vp]
+    lib.convgemm_free_pack_buffs.restype = None
+    lib.convgemm_get_last_error.restype = ci
+    ac, bc = vp(), vp()
+    assert lib.alloc_pack_buffs(ctypes.byref(ac), ctypes.byref(bc)) == 0
+    s = Shape(2, 10, 9, 16, 24, 3, 3, 1, 1, 2, 1, 1, 1)
+    rng = np.random.default_rng(42)
+    g = (s.b, s.h, s.w, s.c)
+    conv = (s.kn, s.kh, s.kw, s.vpad, s.hpad, s.vstr, s.hstr, s.vdil, s.hdil)
+    back = (s.kn, s.kh, s.kw, s.vstr, s.hstr, s.vpad, s.hpad, s.vdil, s.hdil)  # stride before padding
+    for layout in (NHWC, NCHW):
+        X, W, dY, epi = _case(rng, s, layout)
+        dims = g if layout == NHWC else (s.b, s.c, s.h, s.w)
+        fwd = lib.sconvGemmNHWC if layout == NHWC else lib.sconvGemmNCHW
+        bwd = lib.sconvGemmNHWC_back if layout == NHWC else lib.sconvGemmNCHW_back
+        dW_, dX_, dY_ = dev(W), dev(X), dev(dY)
+        e = {k: dev(v) for k, v in epi.items() if k != "relu"}
+        Y = torch.empty(s.y_shape(layout), device=DEV)
+        fwd(b"N", *dims, *conv, dW_.data_ptr(), dX_.data_ptr(), Y.data_ptr(), e["bias"].data_ptr(),
+            e["mean"].data_ptr(), e["inv_std"].data_ptr(), e["gamma"].data_ptr(), e["bnbeta"].data_ptr(),
+            bool(epi["relu"]), ac, bc)
+        assert lib.convgemm_get_last_error() == 0
+        assert rel_err(host(Y), oracle.fwd(layout, s, W, X, **epi))[0] <= TOL
+        gW = torch.empty(s.w_shape(layout), device=DEV)
+        fwd(b"T", *dims, *conv, dY_.data_ptr(), dX_.data_ptr(), gW.data_ptr(), None, None, None, None, None,
+            False, ac, bc)
+        assert rel_err(host(gW), oracle.wgrad(layout, s, dY, X))[0] <= TOL
+        gX = torch.zeros(s.x_shape(layout), device=DEV)
+        bwd(*dims, *back, dW_.data_ptr(), dY_.data_ptr(), gX.data_ptr(), ac, bc)
+        assert lib.convgemm_get_last_error() == 0
+        assert rel_err(host(gX), oracle.dgrad(layout, s, W, dY))[0] <= TOL
+    lib.convgemm_free_pack_buffs(ac, bc)
+
+
 def _fixture_shapes(max_macs=6e7, limit=40):
     """Unique CNN layer shapes from the reference's shape list (tests/test.txt,
     columns h w c b kh kw kn stride padding), at batch 1."""
