@@ -185,6 +185,10 @@ def lib() -> ctypes.CDLL:
               "convgemm_weights_nchw_to_nhwc"):
         getattr(L, f).argtypes = [vp, vp, ci, ci, ci, ci, ci, vp]
         getattr(L, f).restype = ci
+    L.cg_num_tilecfgs.argtypes = [ci]
+    L.cg_num_tilecfgs.restype = ci
+    L.cg_get_tilecfg.argtypes = [ci, ci, ctypes.POINTER(ci * 6)]
+    L.cg_get_tilecfg.restype = ci
     L.cg_force_config.argtypes = [ci, ci]
     L.cg_force_config.restype = None
     L.cg_last_plan.argtypes = [ctypes.POINTER(ci)] * 3
@@ -582,3 +586,14 @@ def weights_nhwc_to_nchw(src, dst, c, kh, kw, kn, esize=4, stream=None):
 
 def weights_nchw_to_nhwc(src, dst, kn, c, kh, kw, esize=4, stream=None):
     _conv_layout("convgemm_weights_nchw_to_nhwc", src, dst, kn, c, kh, kw, esize, stream)
+
+
+def tile_configs(dtype=F32):
+    """Tile table of an operand type: list of (BM, BN, BK, WM, WN, pipelined stages or 0)."""
+    L = lib()
+    out = []
+    for c in range(L.cg_num_tilecfgs(int(dtype))):
+        v = (ctypes.c_int * 6)()
+        if L.cg_get_tilecfg(int(dtype), c, ctypes.byref(v)) == 0:
+            out.append(tuple(v))
+    return out

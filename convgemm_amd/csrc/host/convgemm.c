@@ -328,7 +328,7 @@ void cg_default_plan(int elem, const cg_kparams *base, int *cfg_out, int *splits
         cg_kparams kp = *base;
         cg_fill_tiles(&kp, &t, 1);
         long long tiles = (long long)kp.mtiles * kp.ntiles * kp.ncls_h * kp.ncls_w;
-        int lds = (t.BM + t.BN) * (t.BK * esize_of(elem) + 16) * 2;
+        int lds = t.lp ? (t.BM + t.BN) * 128 * t.lp : (t.BM + t.BN) * (t.BK * esize_of(elem) + 16) * 2;
         int per_cu = 163840 / lds;
         if (per_cu > 4) per_cu = 4;
         if (per_cu < 1) per_cu = 1;

@@ -23,7 +23,7 @@
 #include "../cg_internal.h"
 #include "cg_host.h"
 
-#define CM_F 26
+#define CM_F 27
 #define CM_H 64
 #define CM_NP (CM_F * CM_H + CM_H + CM_H * CM_H + CM_H + CM_H + 1)
 #define OFF_W1 0
@@ -89,13 +89,14 @@ int cg_features(int elem, const cg_kparams *kp0, int cfg, int splits, float *f) 
     f[21] = lg(kp.splits);
     long long wgs = (long long)kp.mtiles * kp.ntiles * kp.splits * kp.ncls_h * kp.ncls_w;
     f[22] = lg((double)wgs);
-    int lds = (t.BM + t.BN) * (t.BK * esz(elem) + 16) * 2;
+    int lds = t.lp ? (t.BM + t.BN) * 128 * t.lp : (t.BM + t.BN) * (t.BK * esz(elem) + 16) * 2;
     int per_cu = 163840 / lds;
     if (per_cu < 1) per_cu = 1;
     if (per_cu > 8) per_cu = 8;
     f[23] = (float)((double)kp.M * kp.N / ((double)kp.mtiles * t.BM * kp.ntiles * t.BN));
     f[24] = lg((double)((wgs + 256LL * per_cu - 1) / (256LL * per_cu)));
     f[25] = lg(2.0 * (double)kp.M * kp.N * (double)kp.KT * t.BK);
+    f[26] = t.lp ? 1.0f : 0.0f;
     return 0;
 }
 
@@ -425,7 +426,7 @@ convgemm_status_t convgemm_costmodel_predict(const convgemm_desc_t *desc, convge
 
 /* -------------------------------------------------------------- files */
 
-static const char MAGIC[8] = {'C', 'G', 'M', 'L', 'P', '0', '0', '1'};
+static const char MAGIC[8] = {'C', 'G', 'M', 'L', 'P', '0', '0', '2'};
 
 convgemm_status_t convgemm_costmodel_save(const char *path) {
     pthread_mutex_lock(&m_lock);

@@ -1,6 +1,7 @@
 // igemm_bf16.hip -- BF16 (v_mfma_f32_32x32x16_bf16) instantiations of the implicit-GEMM kernel and
-// its tile-configuration table (BM, BN, BK, waves along M, waves along N).
+// its tile-configuration table (BM, BN, BK, waves along M, waves along N, pipelined).
 #include "igemm_kernel.hpp"
+#include "lp_kernel.hpp"
 
 using namespace cg;
 
@@ -16,10 +17,20 @@ const cg_tilecfg kCfg[] = {
     {128, 128, 64, 2, 2},  // 6
     {64, 64, 64, 2, 2},  // 7
     {128, 256, 32, 2, 2},  // 8
+    {128, 128, 64, 2, 2, 4},  // 9 pipelined, 4 LDS stages
+    {256, 64, 64, 4, 1, 3},  // 10 pipelined, 3 LDS stages
+    {128, 64, 64, 2, 2, 4},  // 11 pipelined, 4 LDS stages
+    {256, 128, 64, 2, 2, 3},  // 12 pipelined, 3 LDS stages
+    {64, 64, 64, 2, 2, 4},  // 13 pipelined, 4 LDS stages
+    {128, 256, 64, 2, 2, 3},  // 14 pipelined, 3 LDS stages
+    {128, 64, 64, 2, 2, 3},  // 15 pipelined, 3 LDS stages
+    {64, 128, 64, 2, 2, 3},  // 16 pipelined, 3 LDS stages
 };
 constexpr int kNum = sizeof(kCfg) / sizeof(kCfg[0]);
 // configurations that also have element-load (non-16-byte) instantiations
-const bool kNoVec[kNum] = {true, false, false, true, false, false, false, true, false};
+const bool kNoVec[kNum] = {true, false, false, true, false, false, false, true, false, false, false, false, false, false, false, false, false};
+// configurations served by the pipelined forward kernel (lp_kernel.hpp)
+const bool kLp[kNum] = {false, false, false, false, false, false, false, false, false, true, true, true, true, true, true, true, true};
 
 }  // namespace
 
@@ -31,7 +42,7 @@ extern "C" int cg_bf16_get_cfg(int cfg, cg_tilecfg *out) {
     return 0;
 }
 
-extern "C" int cg_bf16_supports(int cfg, int op, int vec) { return supports<uint16_t>(kCfg, kNoVec, kNum, cfg, op, vec); }
+extern "C" int cg_bf16_supports(int cfg, int op, int vec) { return supports<uint16_t>(kCfg, kNoVec, kNum, cfg, op, vec, kLp); }
 
 static int launch_any(int cfg, const cg_kparams *kp, const cg_group *g, int *occ, void *stream) {
     hipStream_t st = (hipStream_t)stream;
@@ -45,6 +56,14 @@ static int launch_any(int cfg, const cg_kparams *kp, const cg_group *g, int *occ
     case 6: return launch_cfg<uint16_t, 128, 128, 64, 2, 2, false>(kp, g, occ, st);
     case 7: return launch_cfg<uint16_t, 64, 64, 64, 2, 2, true>(kp, g, occ, st);
     case 8: return launch_cfg<uint16_t, 128, 256, 32, 2, 2, false>(kp, g, occ, st);
+    case 9: return launch_lp<uint16_t, 128, 128, 2, 2, 4>(kp, g, occ, st);
+    case 10: return launch_lp<uint16_t, 256, 64, 4, 1, 3>(kp, g, occ, st);
+    case 11: return launch_lp<uint16_t, 128, 64, 2, 2, 4>(kp, g, occ, st);
+    case 12: return launch_lp<uint16_t, 256, 128, 2, 2, 3>(kp, g, occ, st);
+    case 13: return launch_lp<uint16_t, 64, 64, 2, 2, 4>(kp, g, occ, st);
+    case 14: return launch_lp<uint16_t, 128, 256, 2, 2, 3>(kp, g, occ, st);
+    case 15: return launch_lp<uint16_t, 128, 64, 2, 2, 3>(kp, g, occ, st);
+    case 16: return launch_lp<uint16_t, 64, 128, 2, 2, 3>(kp, g, occ, st);
     }
     return (int)hipErrorInvalidValue;
 }

@@ -146,7 +146,7 @@ def test_fp8_rejects_backward_and_missing_scales():
 @pytest.mark.parametrize("dtype", ["bf16", "fp8"])
 def test_every_lowp_configuration(dtype):
     elem = cg.BF16 if dtype == "bf16" else cg.FP8_E4M3
-    ncfg = 9 if dtype == "bf16" else 8
+    ncfg = len(cg.tile_configs(elem))
     s = Shape(2, 11, 10, 64, 80, 3, 3, 1, 1, 1, 1, 1, 1)
     rng = np.random.default_rng(5)
     for layout in (NHWC, NCHW):
@@ -168,3 +168,54 @@ def test_every_lowp_configuration(dtype):
                 finally:
                     cg.force_config(-1, 0)
                 assert rel_err(y, ref)[0] <= (2e-5 if dtype == "bf16" else FP8_TOL), (layout, cfg, splits)
+
+
+# Shapes for the pipelined forward kernel (128-byte K steps, 3-4 LDS stages):
+# a single K step, every tail length, ragged channel blocks, padding, stride,
+# dilation, M and N tails.
+LP_SHAPES = [
+    Shape(2, 9, 9, 64, 40, 1, 1, 0, 0, 1, 1, 1, 1),      # bf16: one K step
+    Shape(2, 11, 10, 72, 80, 3, 3, 1, 1, 1, 1, 1, 1),    # ragged channel block (144 B / 72 B)
+    Shape(1, 13, 12, 128, 96, 3, 3, 1, 1, 2, 2, 1, 1),   # stride 2
+    Shape(2, 10, 10, 48, 136, 3, 3, 2, 2, 1, 1, 2, 2),   # dilation, N tail
+    Shape(1, 7, 7, 256, 64, 5, 5, 2, 2, 1, 1, 1, 1),     # 25 taps x 4/2 blocks
+    Shape(3, 6, 5, 32, 16, 1, 1, 0, 0, 1, 1, 1, 1),      # tiny, M tail
+]
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "fp8"])
+@pytest.mark.parametrize("si", range(len(LP_SHAPES)))
+def test_pipelined_forward_configurations(dtype, si):
+    elem = cg.BF16 if dtype == "bf16" else cg.FP8_E4M3
+    lp = [i for i, t in enumerate(cg.tile_configs(elem)) if t[5] > 0]
+    assert lp, "no pipelined configurations"
+    s = LP_SHAPES[si]
+    if dtype == "fp8" and s.c % 16:
+        pytest.skip("FP8 16-byte chunks need C % 16 == 0")
+    rng = np.random.default_rng(50 + si)
+    X = rng.standard_normal(s.x_shape(NHWC)).astype(np.float32)
+    W = (rng.standard_normal(s.w_shape(NHWC)) * 0.1).astype(np.float32)
+    if dtype == "bf16":
+        (Xd, Xq), (Wd, Wq), scales = q_bf16(X), q_bf16(W), None
+    else:
+        Xd, Xq, sx = q_fp8(X)
+        Wd, Wq, sw = q_fp8(W)
+        scales = (sw, sx)
+    epi = dict(bias=(0.1 * rng.standard_normal(s.kn)).astype(np.float32), relu=True)
+    ref = oracle.fwd(NHWC, s, Wq, Xq, **epi)
+    geo = (s.b, s.h, s.w, s.c, s.kn, s.kh, s.kw, s.vpad, s.hpad, s.vstr, s.hstr, s.vdil, s.hdil)
+    d = cg.make_desc(cg.FWD, NHWC, *geo)
+    bias = torch.from_numpy(epi["bias"]).to(DEV)
+    for cfg in lp:
+        for splits in (1, 2, 3):
+            cg.force_config(cfg, splits)
+            try:
+                Y = torch.full(s.y_shape(NHWC), float("nan"), device=DEV)
+                print(f"lp {dtype} shape {si} cfg {cfg} splits {splits}", flush=True)
+                cg.conv_ex(d, Wd, Xd, Y, dtype=elem, scales=scales, bias=bias, relu=True)
+                torch.cuda.synchronize()
+                assert cg.last_plan()[0] == cfg
+            finally:
+                cg.force_config(-1, 0)
+            y = Y.cpu().numpy()
+            assert rel_err(y, ref)[0] <= (2e-5 if dtype == "bf16" else FP8_TOL), (cfg, splits)
