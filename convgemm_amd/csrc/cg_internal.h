@@ -58,6 +58,7 @@ typedef struct {
     int relu;
     float alpha, beta;
     float scale;     /* product of operand scales (FP8), 1 otherwise */
+    int sk;          /* stream-K workgroups (FWD/WGRAD, splits == 1); 0: off */
 } cg_kparams;
 
 /* Tile configurations of the implicit-GEMM kernels. */
@@ -101,6 +102,12 @@ int cg_launch_splitk_reduce_grouped(int op, const cg_group *g, void *stream);
 long long cg_reduce_items(const cg_kparams *kp);
 /* Resident workgroups per CU of a configuration (occupancy), or 0. */
 int cg_igemm_occupancy(int elem, int cfg, const cg_kparams *kp0);
+/* Stream-K: resident workgroups per CU of the stream-K kernel of a
+ * configuration (0 if it has none), and the fix-up of the tiles that several
+ * workgroups share (kp->sk workgroups, partial tiles of BM x BN floats in
+ * kp->ws: [workgroup][first / last piece][BM][BN]). */
+int cg_igemm_sk_occupancy(int elem, int cfg, const cg_kparams *kp0);
+int cg_launch_sk_fixup(const cg_kparams *kp, int BM, int BN, void *stream);
 
 /* ---- explicit transforms (FP32) and copies ---- */
 typedef struct {

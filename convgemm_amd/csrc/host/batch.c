@@ -146,6 +146,7 @@ static convgemm_status_t plan_one(const convgemm_problem_t *q, convgemm_dtype_t 
 
     int cfg, splits;
     cg_choose_plan(elem, &kp, &cfg, &splits);
+    if (splits < 1) splits = 1; /* stream-K plans run as plain tiles in a grouped launch */
     cg_tilecfg t;
     if (cfg < 0 || cg_get_tilecfg(elem, cfg, &t))
         return errf(CONVGEMM_ERR_UNSUPPORTED, "no kernel configuration for a batch problem");

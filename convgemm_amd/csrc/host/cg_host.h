@@ -49,6 +49,10 @@ int cg_esize(int elem);
 /* Device allocation through the library's allocator (honours the test hook). */
 int cg_malloc(void **p, size_t bytes);
 convgemm_status_t cg_hip_fail(int err, const char *what);
+/* Stream-K: workgroups for a problem with tile fields filled (0: none for
+ * this configuration/operation), and the bytes of their partial tiles. */
+int cg_sk_groups(int elem, int cfg, const cg_kparams *kp);
+size_t cg_sk_bytes(int groups, const cg_tilecfg *t);
 /* The library stream of the drop-in calls (hipStream_t). */
 void *cg_get_stream(void);
 

@@ -452,7 +452,7 @@ static int choose_plan(int elem, const cg_kparams *kp, int *cfg, int *splits) {
     *splits = 1;
     if (g_force_cfg >= 0 && cg_tilecfg_supports(elem, g_force_cfg, kp->op, kp->vec)) {
         *cfg = g_force_cfg;
-        *splits = g_force_splits > 0 ? g_force_splits : 1;
+        *splits = g_force_splits >= 0 ? g_force_splits : 1; /* 0: stream-K */
     } else if (g_plan_hook) {
         g_plan_hook(g_plan_hook_ctx, elem, kp, cfg, splits);
     }
@@ -463,6 +463,29 @@ static int choose_plan(int elem, const cg_kparams *kp, int *cfg, int *splits) {
 int cg_choose_plan(int elem, const cg_kparams *kp, int *cfg, int *splits) {
     return choose_plan(elem, kp, cfg, splits);
 }
+
+static int num_cus(void) {
+    static int n = 0;
+    if (!n) {
+        int dev = 0, v = 0;
+        hipGetDevice(&dev);
+        n = (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ? v : 256;
+    }
+    return n;
+}
+
+/* Stream-K workgroups for a problem with tiles filled (0 when the
+ * configuration or operation has no stream-K kernel). */
+int cg_sk_groups(int elem, int cfg, const cg_kparams *kp) {
+    int occ = cg_igemm_sk_occupancy(elem, cfg, kp);
+    if (occ <= 0) return 0;
+    long long g = (long long)occ * num_cus();
+    long long units = (long long)kp->mtiles * kp->ntiles * kp->KT;
+    if (g > units) g = units;
+    return (int)g;
+}
+
+size_t cg_sk_bytes(int groups, const cg_tilecfg *t) { return (size_t)groups * 2 * t->BM * t->BN * sizeof(float); }
 
 static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, const void *in,
                              const void *x, void *out, const convgemm_epilogue_t *epi,
@@ -533,10 +556,16 @@ static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, 
     cg_tilecfg t;
     if (cfg < 0 || cg_get_tilecfg(elem, cfg, &t))
         return set_error(CONVGEMM_ERR_UNSUPPORTED, "no kernel configuration for this problem");
-    cg_fill_tiles(&kp, &t, splits);
+    cg_fill_tiles(&kp, &t, splits > 0 ? splits : 1);
+    /* splits == 0: stream-K (forward and weight gradient, general kernel) */
+    kp.sk = splits == 0 ? cg_sk_groups(elem, cfg, &kp) : 0;
 
-    /* workspace: repacked weights + split-K partials */
-    size_t pbytes = cg_partial_bytes(&kp);
+    /* workspace: repacked weights + split-K partials / stream-K pieces */
+    size_t pbytes = kp.sk ? cg_sk_bytes(kp.sk, &t) : cg_partial_bytes(&kp);
+    if (kp.sk && ws->ws_parts && pbytes > ws->parts_bytes) {
+        kp.sk = 0; /* no room for the pieces: plain tiles */
+        pbytes = 0;
+    }
     while (pbytes && ws->ws_parts && pbytes > ws->parts_bytes && kp.splits > 1) {
         cg_fill_tiles(&kp, &t, kp.splits / 2);
         pbytes = cg_partial_bytes(&kp);
@@ -566,6 +595,10 @@ static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, 
         c.B = nb;
         c.P = (long long)nb * ho * wo;
         cg_fill_tiles(&c, &t, kp.splits);
+        if (kp.sk) { /* as many workgroups as this chunk has units, at most */
+            long long units = (long long)c.mtiles * c.ntiles * c.KT;
+            c.sk = units < kp.sk ? (int)units : kp.sk;
+        }
         c.A = (const char *)x + (size_t)b0 * img_a;
         c.a_bytes = (long long)nb * img_a;
         if (d->op == CONVGEMM_WGRAD) {
@@ -580,14 +613,18 @@ static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, 
         int e = cg_launch_igemm(elem, cfg, &c, stream);
         if (e) return hip_fail((hipError_t)e, "implicit GEMM launch");
         nkern++;
-        if (c.splits > 1) {
+        if (c.sk > 0) {
+            e = cg_launch_sk_fixup(&c, t.BM, t.BN, stream);
+            if (e) return hip_fail((hipError_t)e, "stream-K fix-up");
+            nkern++;
+        } else if (c.splits > 1) {
             e = cg_launch_splitk_reduce(&c, stream);
             if (e) return hip_fail((hipError_t)e, "split-K reduction");
             nkern++;
         }
     }
     t_plan_cfg = cfg;
-    t_plan_splits = kp.splits;
+    t_plan_splits = kp.sk ? 0 : kp.splits;
     t_plan_kernels = nkern;
     return ok();
 }
@@ -629,8 +666,8 @@ convgemm_status_t convgemm_plan(const convgemm_desc_t *d, convgemm_dtype_t dtype
     choose_plan(elem, &kp, &c, &s);
     cg_tilecfg t;
     if (c >= 0 && !cg_get_tilecfg(elem, c, &t)) {
-        cg_fill_tiles(&kp, &t, s);
-        s = kp.splits;
+        cg_fill_tiles(&kp, &t, s > 0 ? s : 1);
+        s = (s == 0 && cg_sk_groups(elem, c, &kp) > 0) ? 0 : kp.splits;
     }
     if (cfg) *cfg = c;
     if (splits) *splits = s;
@@ -659,6 +696,11 @@ size_t convgemm_workspace_size(const convgemm_desc_t *d, convgemm_dtype_t dtype)
         if (cg_get_tilecfg(elem, cfg, &t)) continue;
         cg_fill_tiles(&kp, &t, 32);
         size_t b = cg_partial_bytes(&kp);
+        if (b > parts) parts = b;
+        cg_fill_tiles(&kp, &t, 1);
+        kp.vec = 1;
+        b = cg_sk_bytes(cg_sk_groups(elem, cfg, &kp), &t); /* stream-K pieces */
+        kp.vec = 0;
         if (b > parts) parts = b;
     }
     return wt_al + parts;

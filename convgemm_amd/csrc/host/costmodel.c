@@ -23,7 +23,7 @@
 #include "../cg_internal.h"
 #include "cg_host.h"
 
-#define CM_F 27
+#define CM_F 28
 #define CM_H 64
 #define CM_NP (CM_F * CM_H + CM_H + CM_H * CM_H + CM_H + CM_H + 1)
 #define OFF_W1 0
@@ -70,7 +70,7 @@ int cg_features(int elem, const cg_kparams *kp0, int cfg, int splits, float *f) 
     cg_tilecfg t;
     if (cg_get_tilecfg(elem, cfg, &t)) return -1;
     cg_kparams kp = *kp0;
-    cg_fill_tiles(&kp, &t, splits);
+    cg_fill_tiles(&kp, &t, splits > 0 ? splits : 1);
     memset(f, 0, sizeof(float) * CM_F);
     f[kp.op] = 1.0f;
     f[6 + elem] = 1.0f;
@@ -97,6 +97,12 @@ int cg_features(int elem, const cg_kparams *kp0, int cfg, int splits, float *f) 
     f[24] = lg((double)((wgs + 256LL * per_cu - 1) / (256LL * per_cu)));
     f[25] = lg(2.0 * (double)kp.M * kp.N * (double)kp.KT * t.BK);
     f[26] = t.lp ? 1.0f : 0.0f;
+    if (splits == 0) { /* stream-K: the persistent workgroups, no split partials */
+        const int g = cg_sk_groups(elem, cfg, &kp);
+        f[22] = lg((double)g);
+        f[24] = 0.0f;
+        f[27] = 1.0f;
+    }
     return 0;
 }
 
@@ -426,7 +432,7 @@ convgemm_status_t convgemm_costmodel_predict(const convgemm_desc_t *desc, convge
 
 /* -------------------------------------------------------------- files */
 
-static const char MAGIC[8] = {'C', 'G', 'M', 'L', 'P', '0', '0', '2'};
+static const char MAGIC[8] = {'C', 'G', 'M', 'L', 'P', '0', '0', '3'};
 
 convgemm_status_t convgemm_costmodel_save(const char *path) {
     pthread_mutex_lock(&m_lock);

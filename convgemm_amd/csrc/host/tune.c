@@ -163,6 +163,14 @@ int cg_candidates(int elem, const cg_kparams *base, int *cfgs, int *splits, int 
                 n++;
             }
         }
+        /* stream-K (splits 0) when the tiles do not fill the resident slots evenly */
+        int g = cg_sk_groups(elem, c, &kp);
+        if (g > 0 && (long long)kp.mtiles * kp.ntiles % g != 0 && kp.KT >= 2 &&
+            cg_sk_bytes(g, &t) <= (size_t)CG_PACK_BYTES && n < max) {
+            cfgs[n] = c;
+            splits[n] = 0;
+            n++;
+        }
     }
     return n;
 }

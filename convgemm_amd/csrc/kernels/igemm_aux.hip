@@ -48,6 +48,54 @@ __global__ void __launch_bounds__(256) splitk_reduce_grouped_kernel(const cg_kpa
     }
 }
 
+// ---- stream-K fix-up: one workgroup per tile; tiles that several stream-K
+// workgroups share sum their pieces (workgroup order, deterministic) and go
+// through the epilogue; whole tiles were written back by their workgroup ----
+template <int OP>
+__global__ void __launch_bounds__(256) sk_fixup_kernel(const cg_kparams p, int BM, int BN) {
+    constexpr bool WG = OP == CG_WGRAD_NHWC || OP == CG_WGRAD_NCHW;
+    const int t = blockIdx.x;
+    const long long G = p.sk;
+    const long long U = (long long)p.mtiles * p.ntiles * p.KT;
+    const long long lo = (long long)t * p.KT, hi = lo + p.KT;
+    auto first_unit = [&](long long g) { return g * U / G; };
+    long long g0 = lo * G / U;
+    while (g0 + 1 < G && first_unit(g0 + 1) <= lo) g0++;
+    long long g1 = (hi - 1) * G / U;
+    while (g1 + 1 < G && first_unit(g1 + 1) <= hi - 1) g1++;
+    if (g0 == g1) return;
+    const Ctx x = make_ctx(p, 0);
+    const int nt = t % p.ntiles, mt = t / p.ntiles;
+    int wg_tap = 0, wg_c0 = 0;
+    if (WG) {
+        wg_tap = mt / p.kchunks;
+        wg_c0 = (mt - wg_tap * p.kchunks) * BM;
+    }
+    const long long tile_elems = (long long)BM * BN;
+    for (int e = threadIdx.x; e < BM * BN; e += blockDim.x) {
+        const int ml = e / BN, nl = e - ml * BN;
+        const int n = nt * BN + nl;
+        int m;
+        bool ok;
+        if (WG) {
+            const int c = wg_c0 + ml;
+            m = wg_tap * p.C + c;
+            ok = c < p.C && n < p.KN;
+        } else {
+            m = mt * BM + ml;
+            ok = m < x.Mrows && n < p.KN;
+        }
+        if (!ok) continue;
+        float s = 0.f;
+        for (long long g = g0; g <= g1; g++) {
+            const long long f = first_unit(g);
+            if (first_unit(g + 1) == f) continue;  // an empty range wrote nothing
+            s += p.ws[(g * 2 + (f >= lo ? 0 : 1)) * tile_elems + e];
+        }
+        store_out<OP>(p, x, m, n, s);
+    }
+}
+
 __global__ void scale_kernel(float *out, long long n, float beta) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -171,6 +219,34 @@ extern "C" int cg_igemm_occupancy(int elem, int cfg, const cg_kparams *kp0) {
     case CG_ELEM_FP8: return cg_fp8_occupancy(cfg, kp0);
     }
     return 0;
+}
+
+extern "C" int cg_igemm_sk_occupancy(int elem, int cfg, const cg_kparams *kp0) {
+    cg_tilecfg t;
+    if (cg_get_tilecfg(elem, cfg, &t) || t.lp) return 0;
+    if (kp0->op != CG_FWD_NHWC && kp0->op != CG_FWD_NCHW && kp0->op != CG_WGRAD_NHWC && kp0->op != CG_WGRAD_NCHW)
+        return 0;
+    cg_kparams k = *kp0;
+    k.sk = -1;
+    return cg_igemm_occupancy(elem, cfg, &k);
+}
+
+extern "C" int cg_launch_sk_fixup(const cg_kparams *kp, int BM, int BN, void *stream) {
+    hipStream_t st = (hipStream_t)stream;
+    const unsigned tiles = (unsigned)(kp->mtiles * kp->ntiles);
+    if (tiles == 0) return 0;
+    switch (kp->op) {
+    case CG_FWD_NHWC: hipLaunchKernelGGL(sk_fixup_kernel<CG_FWD_NHWC>, dim3(tiles), dim3(256), 0, st, *kp, BM, BN); break;
+    case CG_FWD_NCHW: hipLaunchKernelGGL(sk_fixup_kernel<CG_FWD_NCHW>, dim3(tiles), dim3(256), 0, st, *kp, BM, BN); break;
+    case CG_WGRAD_NHWC:
+        hipLaunchKernelGGL(sk_fixup_kernel<CG_WGRAD_NHWC>, dim3(tiles), dim3(256), 0, st, *kp, BM, BN);
+        break;
+    case CG_WGRAD_NCHW:
+        hipLaunchKernelGGL(sk_fixup_kernel<CG_WGRAD_NCHW>, dim3(tiles), dim3(256), 0, st, *kp, BM, BN);
+        break;
+    default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
 }
 
 extern "C" long long cg_reduce_items(const cg_kparams *kp) {

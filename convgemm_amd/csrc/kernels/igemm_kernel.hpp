@@ -73,8 +73,12 @@ constexpr int FAR = -(1 << 29);  // row coordinate that fails every bounds test
 
 // One workgroup item of problem p: an output tile, split-K slice and (DGRAD)
 // stride class; `lin` is the item's linear index within the problem.
+// Stream-K (sk_k0 >= 0; FWD/WGRAD): `lin` is the tile, K tiles
+// [sk_k0, sk_k1) are reduced, and the accumulators go to `piece` ([BM][BN],
+// tile-local) unless piece is null (the whole K range: direct write-back).
 template <typename T, int BM, int BN, int BK, int WM, int WN, int OP, bool VEC>
-__device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparams &p, const int lin) {
+__device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparams &p, const int lin, const int sk_k0 = -1,
+                                                                 const int sk_k1 = 0, float *piece = nullptr) {
     typedef typename ET<T>::V V;
     constexpr int VW = ET<T>::VEC;
     constexpr int G = ET<T>::G;
@@ -107,7 +111,8 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
     // once and the small weight panel stays L2-resident (with M fastest the
     // gathered operand was measured re-read once per N tile).
     const int tiles = p.mtiles * p.ntiles;
-    const int tile = lin % tiles, rest = lin / tiles;
+    const bool skm = sk_k0 >= 0;
+    const int tile = skm ? lin : lin % tiles, rest = skm ? 0 : lin / tiles;
     const int split = rest % p.splits, cls = rest / p.splits;
     const int nt = tile % p.ntiles, mt = tile / p.ntiles;
 
@@ -137,8 +142,8 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
     }
     const int KT = WG ? p.KT : nkx * nky * p.kchunks;
     const int per_split = (KT + p.splits - 1) / p.splits;
-    const int kt_begin = min(KT, split * per_split);
-    const int kt_end = min(KT, kt_begin + per_split);
+    const int kt_begin = skm ? sk_k0 : min(KT, split * per_split);
+    const int kt_end = skm ? sk_k1 : min(KT, kt_begin + per_split);
 
     // WGRAD: this M tile is (tap, c0 .. c0+BM)
     int wg_tap = 0, wg_c0 = 0, wg_kx = 0, wg_ky = 0;
@@ -509,6 +514,10 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
                 const int ir = roff + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
                 const int ic = coff + b * 32 + (lane & 31);
                 const int ml = SW ? ic : ir, nl = SW ? ir : ic;
+                if (piece) {  // stream-K partial tile, tile-local
+                    piece[ml * BN + nl] = acc[a][b][v];
+                    continue;
+                }
                 int m;
                 const int n = nt * BN + nl;
                 bool ok;
@@ -539,6 +548,30 @@ __device__ inline int xcd_remap(int bid, int nb) {
 template <typename T, int BM, int BN, int BK, int WM, int WN, int OP, bool VEC>
 __global__ void __launch_bounds__(NT) igemm_kernel(const cg_kparams p) {
     igemm_tile<T, BM, BN, BK, WM, WN, OP, VEC>(p, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// Stream-K (FWD/WGRAD): the tiles x K-tiles of the problem are cut into
+// gridDim.x equal contiguous ranges, one per persistent workgroup (ranges of
+// one XCD adjacent).  A workgroup reduces the K range of each tile its range
+// covers; a whole tile is written back directly, a partial one (only the
+// first and last of a range can be) goes to the workgroup's piece 0 / 1 and
+// sk_fixup_kernel sums the pieces of every shared tile in workgroup order.
+template <typename T, int BM, int BN, int BK, int WM, int WN, int OP, bool VEC>
+__global__ void __launch_bounds__(NT) igemm_sk_kernel(const cg_kparams p) {
+    const int G = gridDim.x;
+    const int g = xcd_remap(blockIdx.x, G);
+    const long long U = (long long)p.mtiles * p.ntiles * p.KT;
+    const long long u_first = (long long)g * U / G, u_end = (long long)(g + 1) * U / G;
+    for (long long u = u_first; u < u_end;) {
+        const int t = (int)(u / p.KT);
+        const int k0 = (int)(u - (long long)t * p.KT);
+        const int k1 = (int)min((long long)p.KT, (long long)k0 + (u_end - u));
+        float *piece = (k0 == 0 && k1 == p.KT) ? nullptr
+                                                : p.ws + ((long long)g * 2 + (u == u_first ? 0 : 1)) * (BM * BN);
+        igemm_tile<T, BM, BN, BK, WM, WN, OP, VEC>(p, t, k0, k1, piece);
+        u += k1 - k0;
+        __syncthreads();  // LDS of this tile is free before the next one starts
+    }
 }
 
 // Many independent problems of one configuration in one launch: item i
@@ -585,9 +618,28 @@ template <typename T> constexpr bool vec_relevant(int op) {
 // nullptr: report resident workgroups per CU instead of launching.
 template <typename T, int BM, int BN, int BK, int WM, int WN, int OP, bool VEC>
 inline int launch_igemm(const cg_kparams *kp, const cg_group *g, int *occ, hipStream_t st) {
+    constexpr bool SKOK = OP == CG_FWD_NHWC || OP == CG_FWD_NCHW || OP == CG_WGRAD_NHWC || OP == CG_WGRAD_NCHW;
     if (occ) {
+        if (kp->sk < 0) {  // occupancy of the stream-K kernel
+            if constexpr (!SKOK) {
+                *occ = 0;
+                return 0;
+            } else {
+                return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    occ, reinterpret_cast<const void *>(&igemm_sk_kernel<T, BM, BN, BK, WM, WN, OP, VEC>), NT, 0);
+            }
+        }
         return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(
             occ, reinterpret_cast<const void *>(&igemm_grouped_kernel<T, BM, BN, BK, WM, WN, OP, VEC>), NT, 0);
+    }
+    if (kp->sk > 0 && !g) {
+        if constexpr (!SKOK) {
+            return (int)hipErrorInvalidValue;
+        } else {
+            hipLaunchKernelGGL((igemm_sk_kernel<T, BM, BN, BK, WM, WN, OP, VEC>), dim3((unsigned)kp->sk), dim3(NT),
+                               0, st, *kp);
+            return (int)hipGetLastError();
+        }
     }
     if (g) {
         if (g->total <= 0) return 0;

@@ -97,10 +97,11 @@ def test_backward_data_accumulates(layout, si):
 CFG_SHAPES = [Shape(2, 11, 10, 24, 40, 3, 3, 1, 1, 1, 1, 1, 1), Shape(1, 13, 12, 20, 36, 3, 3, 1, 0, 2, 2, 1, 1)]
 
 
-@pytest.mark.parametrize("splits", [1, 3])
+@pytest.mark.parametrize("splits", [0, 1, 3])
 @pytest.mark.parametrize("cfg", range(12))
 def test_every_tile_configuration(cfg, splits):
-    """Force each kernel instantiation (and the split-K reduction) on every op.
+    """Force each kernel instantiation (and the split-K reduction; splits 0 =
+    stream-K with its fix-up, forward and weight gradient) on every op.
     A configuration an operation does not offer falls back to the planner."""
     cg.force_config(cfg, splits)
     try:
@@ -196,7 +197,7 @@ def test_deterministic_bitwise(layout):
     s = Shape(4, 14, 14, 64, 64, 3, 3, 1, 1, 1, 1, 1, 1)
     rng = np.random.default_rng(8)
     X, W, dY, epi = _case(rng, s, layout)
-    for cfg, splits in ((-1, 0), (3, 4)):
+    for cfg, splits in ((-1, 0), (3, 4), (3, 0), (0, 0)):
         cg.force_config(cfg, splits)
         a = run_fwd(layout, s, W, X, epi), run_wgrad(layout, s, dY, X), run_dgrad(
             layout, s, W, dY, np.zeros(s.x_shape(layout), np.float32))
@@ -336,3 +337,33 @@ def test_cnn_layer_shape_fixture(si):
         assert rel_err(run_wgrad(layout, s, dY, X), oracle.wgrad(layout, s, dY, X))[0] <= TOL
         dX0 = np.zeros(s.x_shape(layout), np.float32)
         assert rel_err(run_dgrad(layout, s, W, dY, dX0), oracle.dgrad(layout, s, W, dY))[0] <= TOL
+
+
+SK_SHAPES = [
+    Shape(4, 14, 14, 64, 64, 3, 3, 1, 1, 1, 1, 1, 1),    # several tiles per workgroup range
+    Shape(2, 9, 7, 36, 20, 3, 3, 1, 1, 2, 1, 1, 1),      # fewer units than workgroups
+    Shape(8, 28, 28, 32, 96, 1, 1, 0, 0, 1, 1, 1, 1),    # 1x1, ragged N tiles
+    Shape(3, 16, 16, 40, 24, 5, 5, 2, 2, 1, 1, 1, 1),    # long K per tile
+]
+
+
+@pytest.mark.parametrize("cfg", [0, 3, 5, 6, 11])
+@pytest.mark.parametrize("si", range(len(SK_SHAPES)))
+def test_stream_k(cfg, si):
+    """Stream-K (splits 0): forward with the full epilogue and weight gradient,
+    both layouts, against the oracle; the plan reports splits 0 and two kernels."""
+    s = SK_SHAPES[si]
+    cg.force_config(cfg, 0)
+    try:
+        for layout in (NHWC, NCHW):
+            rng = np.random.default_rng(900 + 10 * si + cfg)
+            X, W, dY, epi = _case(rng, s, layout)
+            y = run_fwd(layout, s, W, X, epi)
+            plan = cg.last_plan()
+            assert plan[1] == 0 and plan[2] >= 2, plan
+            assert rel_err(y, oracle.fwd(layout, s, W, X, **epi))[0] <= TOL
+            dw = run_wgrad(layout, s, dY, X)
+            assert cg.last_plan()[1] == 0
+            assert rel_err(dw, oracle.wgrad(layout, s, dY, X))[0] <= TOL
+    finally:
+        cg.force_config(-1, 0)
