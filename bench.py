@@ -155,6 +155,71 @@ def precision_sweep(torch, cg, stream, tune=True, reps=10, batch=64):
     return res
 
 
+def backward_sweep(torch, cg, stream, reps=10, batch=32):
+    """Backward of the layer sweep at batch 32, NHWC: weight gradient
+    dW = im2row(X)^T dY and backward data dX += row2im(W^T dY), FP32 and BF16
+    operands (FP32 accumulation), autotuned; TFLOP/s of the convolution
+    kernels and the error against the FP64 oracle at sampled outputs (BF16
+    against the unquantized FP32 data)."""
+    from oracle import Shape, sampled
+    res = {}
+    for li, (name, spec) in enumerate(LAYERS):
+        _, h, w, c, kn, kh, kw, pad, st = spec
+        s = Shape(batch, h, w, c, kn, kh, kw, pad, pad, st, st, 1, 1)
+        flops = layer_flops((batch,) + spec[1:])
+        g = torch.Generator(device="cuda").manual_seed(0xC0FFEE + 3000 + li)
+        X = torch.randn(s.x_shape(0), device="cuda", generator=g)
+        W = torch.randn(s.w_shape(0), device="cuda", generator=g) * (2.0 / s.K) ** 0.5
+        dY = torch.randn(s.y_shape(0), device="cuda", generator=g)
+        rng = np.random.default_rng(li)
+        ws = np.stack([rng.integers(0, n, 96) for n in (s.c, s.kh, s.kw, s.kn)], 1)
+        xs = np.stack([rng.integers(0, n, 96) for n in (s.b, s.h, s.w, s.c)], 1)
+        Xh, Wh, dYh = X.cpu().numpy(), W.cpu().numpy(), dY.cpu().numpy()
+        ref_w = sampled.wgrad_sampled(0, s, dYh, Xh, ws)
+        ref_x = sampled.dgrad_sampled(0, s, Wh, dYh, xs)
+        row = {}
+        for dt, dname in ((cg.F32, "fp32"), (cg.BF16, "bf16")):
+            if dt == cg.F32:
+                Xo, Wo, dYo = X, W, dY
+            else:
+                Xo, Wo, dYo = (torch.empty(t.shape, dtype=torch.int16, device="cuda") for t in (X, W, dY))
+                for src, dst in ((X, Xo), (W, Wo), (dY, dYo)):
+                    cg.quantize(src, dst, cg.BF16, stream=stream)
+            dW = torch.empty(s.w_shape(0), device="cuda")
+            dX = torch.zeros(s.x_shape(0), device="cuda")
+            dw_desc = cg.make_desc(cg.WGRAD, 0, batch, h, w, c, kn, kh, kw, pad, pad, st, st)
+            dx_desc = cg.make_desc(cg.DGRAD, 0, batch, h, w, c, kn, kh, kw, pad, pad, st, st)
+            out = {}
+            for op, desc, args in (("wgrad", dw_desc, (dYo, Xo, dW)), ("dgrad", dx_desc, (Wo, dYo, dX))):
+                cg.autotune(desc, *args, dtype=dt, reps=3, stream=stream)
+                for _ in range(2):
+                    cg.conv_ex(desc, *args, beta=1.0 if op == "dgrad" else 0.0, dtype=dt, stream=stream)
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(reps):
+                    cg.conv_ex(desc, *args, beta=1.0 if op == "dgrad" else 0.0, dtype=dt, stream=stream)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / reps
+                out[op] = {"ms": round(ms, 4), "tflops": round(flops / (ms * 1e-3) / 1e12, 2),
+                           "plan": list(cg.last_plan()[:2])}
+            # errors of one clean evaluation of each
+            cg.conv_ex(dw_desc, dYo, Xo, dW, dtype=dt, stream=stream)
+            dX.zero_()
+            cg.conv_ex(dx_desc, Wo, dYo, dX, beta=1.0, dtype=dt, stream=stream)
+            torch.cuda.synchronize()
+            gw = sampled.w_index(0, dW.cpu().numpy(), ws)
+            gx = sampled.nhwc_index(0, dX.cpu().numpy(), xs)
+            out["wgrad"]["fp64_max_rel_err"] = float(np.max(np.abs(gw - ref_w)) / np.max(np.abs(ref_w)))
+            out["dgrad"]["fp64_max_rel_err"] = float(np.max(np.abs(gx - ref_x)) / np.max(np.abs(ref_x)))
+            row[dname] = out
+        for op in ("wgrad", "dgrad"):
+            row["bf16"][op]["speedup_vs_fp32"] = round(row["fp32"][op]["ms"] / row["bf16"][op]["ms"], 2)
+        res[name + "_b32_NHWC"] = row
+    return res
+
+
 def batch_sweep(torch, cg, stream, n=256, reps=10, seed=5):
     """Many uneven BF16 forward layers (NHWC; batch {1,4,16}, H=W {7..56},
     Cin/Cout 32..1024, k {1,3,5}, s {1,2}): one conv_ex call per problem vs
@@ -279,6 +344,7 @@ def main(argv=None):
     ap.add_argument("--no-tune", action="store_true", help="use the analytic planner instead of autotuning at setup")
     ap.add_argument("--no-precision", action="store_true", help="skip the BF16/FP8 precision sweep")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-layer sweep")
+    ap.add_argument("--no-backward", action="store_true", help="skip the backward (wgrad/dgrad) sweep")
     ap.add_argument("--tune-db", default="", help="tuning database: loaded if present (layers found there are "
                     "not re-tuned), written back after tuning")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -408,6 +474,8 @@ def main(argv=None):
     }
     if not a.no_precision:
         out["precision_sweep"] = precision_sweep(torch, cg, stream, tune=not a.no_tune)
+    if not a.no_backward:
+        out["backward_sweep"] = backward_sweep(torch, cg, stream)
     if not a.no_batch:
         out["batch_sweep"] = batch_sweep(torch, cg, stream)
     if not a.no_cpu_baseline:

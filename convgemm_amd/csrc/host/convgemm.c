@@ -332,8 +332,13 @@ void cg_default_plan(int elem, const cg_kparams *base, int *cfg_out, int *splits
         int per_cu = 163840 / lds;
         if (per_cu > 4) per_cu = 4;
         if (per_cu < 1) per_cu = 1;
-        for (int s = 1; s <= 32; s *= 2) {
+        for (int s = 1; s <= 256; s *= 2) {
             if (s > 1 && kp.KT / s < 2) break;
+            if (s > 1) {
+                cg_kparams k2 = kp;
+                cg_fill_tiles(&k2, &t, s);
+                if (cg_partial_bytes(&k2) > (size_t)CG_PACK_BYTES) break;
+            }
             long long wgs = tiles * s;
             long long slots = (long long)CUS * per_cu;
             double waves = (double)((wgs + slots - 1) / slots);
@@ -681,7 +686,8 @@ size_t convgemm_workspace_size(const convgemm_desc_t *d, convgemm_dtype_t dtype)
     size_t wt = weight_repack(d, elem, &d0, &d1, &d2, &perm)
                     ? (size_t)d->c * d->kh * d->kw * d->kn * esize_of(elem) : 0;
     size_t wt_al = (wt + 255) & ~(size_t)255;
-    /* the largest partial buffer any plan may use: 32 splits */
+    /* the largest partial buffer any plan may use (plans keep split-K
+     * partials and stream-K pieces within CG_PACK_BYTES) */
     cg_kparams kp;
     memset(&kp, 0, sizeof(kp));
     kp.op = opkind(d);
@@ -694,8 +700,9 @@ size_t convgemm_workspace_size(const convgemm_desc_t *d, convgemm_dtype_t dtype)
     for (int cfg = 0; cfg < n; cfg++) {
         cg_tilecfg t;
         if (cg_get_tilecfg(elem, cfg, &t)) continue;
-        cg_fill_tiles(&kp, &t, 32);
+        cg_fill_tiles(&kp, &t, 256);
         size_t b = cg_partial_bytes(&kp);
+        if (b > (size_t)CG_PACK_BYTES) b = CG_PACK_BYTES;
         if (b > parts) parts = b;
         cg_fill_tiles(&kp, &t, 1);
         kp.vec = 1;

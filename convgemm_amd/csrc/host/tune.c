@@ -144,7 +144,7 @@ int cg_tune_foreach(void (*fn)(void *ctx, const int *key, int n, const int *cfg,
 /* ------------------------------------------------------------ candidates */
 
 int cg_candidates(int elem, const cg_kparams *base, int *cfgs, int *splits, int max) {
-    static const int S[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32};
+    static const int S[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256};
     int n = 0, ncfg = cg_num_tilecfgs(elem);
     for (int c = 0; c < ncfg; c++) {
         cg_tilecfg t;
