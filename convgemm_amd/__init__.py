@@ -40,7 +40,8 @@ EXPORTED_SYMBOLS = (
     "alloc_pack_buffs", "sconvGemmNHWC", "sconvGemmNHWC_back", "sconvGemmNCHW", "sconvGemmNCHW_back",
     "convgemm_conv_ex", "convgemm_workspace_size", "convgemm_out_size", "convgemm_free_pack_buffs",
     "convgemm_set_stream", "convgemm_get_last_error", "convgemm_last_error_message", "convgemm_status_string",
-    "convgemm_version", "convgemm_quantize",
+    "convgemm_version", "convgemm_quantize", "convgemm_packed_weights_size", "convgemm_pack_weights",
+    "convgemm_conv_packed",
     # convgemm_tune.h
     "convgemm_set_planner", "convgemm_autotune", "convgemm_tune_count", "convgemm_tune_save", "convgemm_tune_load",
     "convgemm_tune_clear", "convgemm_plan", "convgemm_costmodel_train", "convgemm_costmodel_save",
@@ -111,6 +112,12 @@ def lib() -> ctypes.CDLL:
     L.convgemm_conv_ex.argtypes = [ctypes.POINTER(Desc), cf, cf, vp, vp, vp, ctypes.POINTER(Epilogue), ci, vp, vp,
                                    ctypes.c_size_t, vp]
     L.convgemm_conv_ex.restype = ci
+    L.convgemm_packed_weights_size.argtypes = [ctypes.POINTER(Desc), ci]
+    L.convgemm_packed_weights_size.restype = ctypes.c_size_t
+    L.convgemm_pack_weights.argtypes = [ctypes.POINTER(Desc), ci, vp, vp, vp]
+    L.convgemm_pack_weights.restype = ci
+    L.convgemm_conv_packed.argtypes = L.convgemm_conv_ex.argtypes
+    L.convgemm_conv_packed.restype = ci
     L.convgemm_workspace_size.argtypes = [ctypes.POINTER(Desc), ci]
     L.convgemm_workspace_size.restype = ctypes.c_size_t
     L.convgemm_out_size.argtypes = [ci] * 5
@@ -597,3 +604,28 @@ def tile_configs(dtype=F32):
         if L.cg_get_tilecfg(int(dtype), c, ctypes.byref(v)) == 0:
             out.append(tuple(v))
     return out
+
+
+def packed_weights_size(desc: Desc, dtype=F32) -> int:
+    """Bytes of the prepacked weights of this problem (0: weights are read as given)."""
+    return int(lib().convgemm_packed_weights_size(ctypes.byref(desc), int(dtype)))
+
+
+def pack_weights(desc: Desc, in_, packed, dtype=F32, stream=None) -> None:
+    """Repack the weights once into `packed` for conv_packed."""
+    L = lib()
+    st = L.convgemm_pack_weights(ctypes.byref(desc), int(dtype), _ptr(in_), _ptr(packed), _stream_handle(stream))
+    if st != 0:
+        raise ConvGemmError(st, L.convgemm_last_error_message().decode())
+
+
+def conv_packed(desc: Desc, packed, x, out, alpha=1.0, beta=0.0, bias=None, mean=None, inv_std=None, gamma=None,
+                bnbeta=None, relu=False, dtype=F32, scales=None, workspace=None, ws_bytes=0, stream=None):
+    """conv_ex on weights prepacked by pack_weights (no repack kernel per call)."""
+    L = lib()
+    epi = Epilogue(_ptr(bias), _ptr(mean), _ptr(inv_std), _ptr(gamma), _ptr(bnbeta), int(bool(relu)))
+    st = L.convgemm_conv_packed(ctypes.byref(desc), float(alpha), float(beta), _ptr(packed), _ptr(x), _ptr(out),
+                                ctypes.byref(epi), int(dtype), _scales_ptr(scales), _ptr(workspace), int(ws_bytes),
+                                _stream_handle(stream))
+    if st != 0:
+        raise ConvGemmError(st, L.convgemm_last_error_message().decode())

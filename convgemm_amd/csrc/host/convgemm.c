@@ -364,6 +364,7 @@ typedef struct {
     size_t parts_bytes;
     void *ws_w;
     size_t w_bytes;
+    int prepacked; /* `in` already holds the weights in the kernels' order */
 } cg_wsets;
 
 /* Weight order the kernels read, and the repack that produces it
@@ -372,7 +373,7 @@ static int weight_repack(const convgemm_desc_t *d, int elem, int *d0, int *d1, i
     int taps = d->kh * d->kw;
     int K = d->c * taps;
     if (d->op == CONVGEMM_WGRAD) return 0;
-    if (d->op == CONVGEMM_FWD && elem != CG_ELEM_F32) {
+    if (d->op == CONVGEMM_FWD) {
         if (d->layout == CONVGEMM_NHWC) { /* [c][tap][kn] -> [kn][tap][c] */
             *d0 = d->c; *d1 = taps; *d2 = d->kn; *perm = 0;
         } else { /* [kn][c][tap] -> [kn][tap][c] */
@@ -498,7 +499,7 @@ static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, 
                              hipStream_t stream) {
     cg_kparams kp;
     int elem, chunk;
-    convgemm_status_t st = cg_prepare(d, dtype, in, x, ws->ws_w, &kp, &elem, &chunk);
+    convgemm_status_t st = cg_prepare(d, dtype, in, x, ws->prepacked ? in : ws->ws_w, &kp, &elem, &chunk);
     if (st != CONVGEMM_SUCCESS) return st;
     const int es = esize_of(elem);
     const int ho = kp.Ho, wo = kp.Wo;
@@ -543,7 +544,7 @@ static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, 
 
     /* weights in the order the kernels read */
     int rd0 = 0, rd1 = 0, rd2 = 0, rperm = 0;
-    const int repack = weight_repack(d, elem, &rd0, &rd1, &rd2, &rperm);
+    const int repack = weight_repack(d, elem, &rd0, &rd1, &rd2, &rperm) && !ws->prepacked;
     const size_t wt_bytes = repack ? (size_t)K * d->kn * es : 0;
     if (repack && ws->ws_w && ws->w_bytes < wt_bytes)
         return set_error(CONVGEMM_ERR_WORKSPACE, "weight workspace %zu < %zu bytes", ws->w_bytes, wt_bytes);
@@ -632,6 +633,41 @@ static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, 
     t_plan_splits = kp.sk ? 0 : kp.splits;
     t_plan_kernels = nkern;
     return ok();
+}
+
+size_t convgemm_packed_weights_size(const convgemm_desc_t *d, convgemm_dtype_t dtype) {
+    int ho, wo, d0, d1, d2, perm;
+    if (validate(d, &ho, &wo) != CONVGEMM_SUCCESS) return 0;
+    const int elem = elem_of(dtype);
+    if (!weight_repack(d, elem, &d0, &d1, &d2, &perm)) return 0;
+    return (size_t)d->c * d->kh * d->kw * d->kn * esize_of(elem);
+}
+
+convgemm_status_t convgemm_pack_weights(const convgemm_desc_t *d, convgemm_dtype_t dtype, const void *in,
+                                        void *packed, convgemm_stream_t stream) {
+    int ho, wo, d0, d1, d2, perm;
+    convgemm_status_t st = validate(d, &ho, &wo);
+    if (st != CONVGEMM_SUCCESS) return st;
+    if (dtype != CONVGEMM_F32 && dtype != CONVGEMM_BF16 && dtype != CONVGEMM_FP8_E4M3)
+        return set_error(CONVGEMM_ERR_INVALID_VALUE, "unknown dtype %d", (int)dtype);
+    const int elem = elem_of(dtype);
+    if (!weight_repack(d, elem, &d0, &d1, &d2, &perm)) return ok(); /* read as given: nothing to pack */
+    if (!in || !packed) return set_error(CONVGEMM_ERR_INVALID_VALUE, "NULL weights or packed buffer");
+    int e = cg_launch_permute3(in, packed, d0, d1, d2, perm, esize_of(elem), stream);
+    return e ? hip_fail((hipError_t)e, "weight repack") : ok();
+}
+
+convgemm_status_t convgemm_conv_packed(const convgemm_desc_t *desc, float alpha, float beta, const void *packed,
+                                       const void *x, void *out, const convgemm_epilogue_t *epi,
+                                       convgemm_dtype_t dtype, const float *scales, void *workspace,
+                                       size_t ws_bytes, convgemm_stream_t stream) {
+    cg_wsets ws = {0};
+    ws.prepacked = 1;
+    if (workspace && ws_bytes) {
+        ws.ws_parts = workspace;
+        ws.parts_bytes = ws_bytes;
+    }
+    return run(desc, alpha, beta, packed, x, out, epi, dtype, scales, &ws, (hipStream_t)stream);
 }
 
 convgemm_status_t convgemm_conv_ex(const convgemm_desc_t *desc, float alpha, float beta, const void *in,

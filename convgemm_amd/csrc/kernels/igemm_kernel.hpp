@@ -36,7 +36,7 @@ template <int OP> constexpr bool swap_out() {
 // Forward weights: FP32 reads the caller's [c][kh][kw][kn] order (rows = kn
 // walk contiguous memory); BF16/FP8 read a [kn][kh][kw][c] repack so that a
 // 16-byte chunk of 8/16 channels is one vector load.
-template <typename T> constexpr bool fwd_b_kmajor() { return !std::is_same<T, float>::value; }
+template <typename T> constexpr bool fwd_b_kmajor() { return true; }
 
 // How each operand tile is read from HBM:
 //   KV: contiguous along K, one 16-byte chunk per load (VEC), else element loads

@@ -146,6 +146,25 @@ convgemm_status_t convgemm_conv_ex(const convgemm_desc_t *desc, float alpha, flo
                                    const float *scales,
                                    void *workspace, size_t ws_bytes, convgemm_stream_t stream);
 
+/* Weight prepacking for repeated calls with the same weights (no counterpart
+ * in the reference, which packs on every call).  Some operations read the
+ * weights in another order than the caller's (BF16/FP8/FP32 forward:
+ * [kn][kh][kw][c]; NCHW backward data: [c][kh][kw][kn]); every call of
+ * convgemm_conv_ex then starts with a repack kernel.  convgemm_pack_weights
+ * does that repack once into `packed` (convgemm_packed_weights_size bytes;
+ * 0 means the operation reads the weights as given), and convgemm_conv_packed
+ * is convgemm_conv_ex on such packed weights (`workspace` then only holds
+ * split-K partials).  Packed weights stay valid while the weights and the
+ * descriptor's shape are unchanged. */
+size_t convgemm_packed_weights_size(const convgemm_desc_t *desc, convgemm_dtype_t dtype);
+convgemm_status_t convgemm_pack_weights(const convgemm_desc_t *desc, convgemm_dtype_t dtype, const void *in,
+                                        void *packed, convgemm_stream_t stream);
+convgemm_status_t convgemm_conv_packed(const convgemm_desc_t *desc, float alpha, float beta,
+                                       const void *packed, const void *x, void *out,
+                                       const convgemm_epilogue_t *epi, convgemm_dtype_t dtype,
+                                       const float *scales, void *workspace, size_t ws_bytes,
+                                       convgemm_stream_t stream);
+
 /* Convert n FP32 values on the device to `dtype`, asynchronously on `stream`.
  * CONVGEMM_BF16: round to nearest even, *scale_out = 1.
  * CONVGEMM_FP8_E4M3: per-tensor amax scaling, scale = amax/448 (1 if amax is

@@ -367,3 +367,45 @@ def test_stream_k(cfg, si):
             assert rel_err(dw, oracle.wgrad(layout, s, dY, X))[0] <= TOL
     finally:
         cg.force_config(-1, 0)
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "fp8"])
+def test_prepacked_weights_bit_identical(dtype):
+    """pack_weights once + conv_packed gives the bits of conv_ex (same plan), for
+    every operation that repacks (forward, NCHW backward data) and the ones that
+    read weights as given (packed size 0)."""
+    elem = {"f32": cg.F32, "bf16": cg.BF16, "fp8": cg.FP8_E4M3}[dtype]
+    s = Shape(2, 12, 11, 32, 40, 3, 3, 1, 1, 2, 1, 1, 1)
+    g = torch.Generator().manual_seed(77)
+    tdt = {"f32": torch.float32, "bf16": torch.bfloat16, "fp8": torch.float8_e4m3fn}[dtype]
+
+    def operand(shape):
+        t = torch.randn(shape, generator=g).to(tdt)
+        if dtype == "bf16":
+            t = t.view(torch.int16)
+        elif dtype == "fp8":
+            t = t.view(torch.uint8)
+        return t.to(DEV)
+
+    scales = (0.5, 2.0) if dtype == "fp8" else None
+    ops = [(cg.FWD, NHWC), (cg.FWD, NCHW)] + ([] if dtype == "fp8" else [(cg.DGRAD, NHWC), (cg.DGRAD, NCHW)])
+    for op, layout in ops:
+        d = cg.make_desc(op, layout, s.b, s.h, s.w, s.c, s.kn, s.kh, s.kw, s.vpad, s.hpad, s.vstr, s.hstr)
+        W = operand(s.w_shape(layout))
+        Xin = operand(s.x_shape(layout) if op == cg.FWD else s.y_shape(layout))
+        out_shape = s.y_shape(layout) if op == cg.FWD else s.x_shape(layout)
+        y0 = torch.zeros(out_shape, device=DEV)
+        y1 = torch.zeros(out_shape, device=DEV)
+        beta = 1.0 if op == cg.DGRAD else 0.0
+        cg.conv_ex(d, W, Xin, y0, beta=beta, dtype=elem, scales=scales)
+        torch.cuda.synchronize()
+        plan = cg.last_plan()
+        n = cg.packed_weights_size(d, elem)
+        packed = torch.empty(max(n, 1), dtype=torch.uint8, device=DEV) if n else W
+        if n:
+            cg.pack_weights(d, W, packed, dtype=elem)
+        cg.conv_packed(d, packed, Xin, y1, beta=beta, dtype=elem, scales=scales)
+        torch.cuda.synchronize()
+        kplan = cg.last_plan()
+        assert kplan[:2] == plan[:2] and kplan[2] == plan[2] - (1 if n else 0), (plan, kplan)
+        assert torch.equal(y0, y1), (op, layout)
