@@ -33,9 +33,9 @@ template <int OP> constexpr bool is_wgrad() { return OP == CG_WGRAD_NHWC || OP =
 template <int OP> constexpr bool swap_out() {
     return OP == CG_FWD_NCHW || OP == CG_DGRAD_NCHW || OP == CG_WGRAD_NCHW;
 }
-// Forward weights: FP32 reads the caller's [c][kh][kw][kn] order (rows = kn
-// walk contiguous memory); BF16/FP8 read a [kn][kh][kw][c] repack so that a
-// 16-byte chunk of 8/16 channels is one vector load.
+// Forward weights: every type reads a [kn][kh][kw][c] repack, so that a
+// 16-byte chunk of 4/8/16 channels is one vector load.  Returning false makes
+// the loader read the caller's [c][kh][kw][kn] order in place (lanes walk kn).
 template <typename T> constexpr bool fwd_b_kmajor() { return true; }
 
 // How each operand tile is read from HBM:
