@@ -377,6 +377,7 @@ static int weight_repack(const convgemm_desc_t *d, int elem, int *d0, int *d1, i
         if (d->layout == CONVGEMM_NHWC) { /* [c][tap][kn] -> [kn][tap][c] */
             *d0 = d->c; *d1 = taps; *d2 = d->kn; *perm = 0;
         } else { /* [kn][c][tap] -> [kn][tap][c] */
+            if (taps == 1) return 0; /* a 1x1 kernel is already in that order */
             *d0 = d->kn; *d1 = d->c; *d2 = taps; *perm = 1;
         }
         return 1;

@@ -1,6 +1,7 @@
 // igemm_f32.hip -- FP32 (v_mfma_f32_32x32x2f32) instantiations of the implicit-GEMM kernel and
-// its tile-configuration table (BM, BN, BK, waves along M, waves along N).
+// its tile-configuration table (BM, BN, BK, waves along M, waves along N, pipelined).
 #include "igemm_kernel.hpp"
+#include "lp_kernel.hpp"
 
 using namespace cg;
 
@@ -19,10 +20,24 @@ const cg_tilecfg kCfg[] = {
     {64, 256, 16, 1, 4},  // 9
     {256, 128, 16, 2, 2},  // 10
     {128, 64, 16, 2, 2},  // 11
+    {128, 128, 32, 2, 2, 3},  // 12 pipelined, 3 LDS stages
+    {128, 128, 32, 2, 2, 2},  // 13 pipelined, 2 LDS stages
+    {256, 64, 32, 4, 1, 3},  // 14 pipelined, 3 LDS stages
+    {128, 64, 32, 2, 2, 4},  // 15 pipelined, 4 LDS stages
+    {64, 128, 32, 2, 2, 4},  // 16 pipelined, 4 LDS stages
+    {256, 128, 32, 2, 2, 3},  // 17 pipelined, 3 LDS stages
+    {128, 256, 32, 2, 2, 3},  // 18 pipelined, 3 LDS stages
+    {64, 64, 32, 2, 2, 4},  // 19 pipelined, 4 LDS stages
+    {256, 64, 32, 4, 1, 2},  // 20 pipelined, 2 LDS stages
+    {128, 64, 32, 2, 2, 2},  // 21 pipelined, 2 LDS stages
 };
 constexpr int kNum = sizeof(kCfg) / sizeof(kCfg[0]);
 // configurations that also have element-load (non-16-byte) instantiations
-const bool kNoVec[kNum] = {true, false, false, true, false, false, true, false, false, false, false, false};
+const bool kNoVec[kNum] = {true, false, false, true, false, false, true, false, false, false, false, false,
+                           false, false, false, false, false, false, false, false, false, false};
+// configurations served by the pipelined forward kernel (lp_kernel.hpp)
+const bool kLp[kNum] = {false, false, false, false, false, false, false, false, false, false, false, false,
+                        true, true, true, true, true, true, true, true, true, true};
 
 }  // namespace
 
@@ -34,7 +49,7 @@ extern "C" int cg_f32_get_cfg(int cfg, cg_tilecfg *out) {
     return 0;
 }
 
-extern "C" int cg_f32_supports(int cfg, int op, int vec) { return supports<float>(kCfg, kNoVec, kNum, cfg, op, vec); }
+extern "C" int cg_f32_supports(int cfg, int op, int vec) { return supports<float>(kCfg, kNoVec, kNum, cfg, op, vec, kLp); }
 
 static int launch_any(int cfg, const cg_kparams *kp, const cg_group *g, int *occ, void *stream) {
     hipStream_t st = (hipStream_t)stream;
@@ -51,6 +66,16 @@ static int launch_any(int cfg, const cg_kparams *kp, const cg_group *g, int *occ
     case 9: return launch_cfg<float, 64, 256, 16, 1, 4, false>(kp, g, occ, st);
     case 10: return launch_cfg<float, 256, 128, 16, 2, 2, false>(kp, g, occ, st);
     case 11: return launch_cfg<float, 128, 64, 16, 2, 2, false>(kp, g, occ, st);
+    case 12: return launch_lp<float, 128, 128, 2, 2, 3>(kp, g, occ, st);
+    case 13: return launch_lp<float, 128, 128, 2, 2, 2>(kp, g, occ, st);
+    case 14: return launch_lp<float, 256, 64, 4, 1, 3>(kp, g, occ, st);
+    case 15: return launch_lp<float, 128, 64, 2, 2, 4>(kp, g, occ, st);
+    case 16: return launch_lp<float, 64, 128, 2, 2, 4>(kp, g, occ, st);
+    case 17: return launch_lp<float, 256, 128, 2, 2, 3>(kp, g, occ, st);
+    case 18: return launch_lp<float, 128, 256, 2, 2, 3>(kp, g, occ, st);
+    case 19: return launch_lp<float, 64, 64, 2, 2, 4>(kp, g, occ, st);
+    case 20: return launch_lp<float, 256, 64, 4, 1, 2>(kp, g, occ, st);
+    case 21: return launch_lp<float, 128, 64, 2, 2, 2>(kp, g, occ, st);
     }
     return (int)hipErrorInvalidValue;
 }

@@ -24,12 +24,18 @@ const cg_tilecfg kCfg[] = {
     {128, 256, 128, 2, 2, 3},  // 13 pipelined, 3 LDS stages
     {128, 64, 128, 2, 2, 3},  // 14 pipelined, 3 LDS stages
     {64, 128, 128, 2, 2, 3},  // 15 pipelined, 3 LDS stages
+    {128, 128, 128, 2, 2, 2},  // 16 pipelined, 2 LDS stages
+    {256, 64, 128, 4, 1, 2},  // 17 pipelined, 2 LDS stages
+    {128, 64, 128, 2, 2, 2},  // 18 pipelined, 2 LDS stages
+    {64, 128, 128, 2, 2, 2},  // 19 pipelined, 2 LDS stages
+    {256, 128, 128, 2, 2, 2},  // 20 pipelined, 2 LDS stages
+    {64, 64, 128, 2, 2, 2},  // 21 pipelined, 2 LDS stages
 };
 constexpr int kNum = sizeof(kCfg) / sizeof(kCfg[0]);
 // configurations that also have element-load (non-16-byte) instantiations
-const bool kNoVec[kNum] = {true, false, false, true, false, false, false, false, false, false, false, false, false, false, false, false};
+const bool kNoVec[kNum] = {true, false, false, true, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false};
 // configurations served by the pipelined forward kernel (lp_kernel.hpp)
-const bool kLp[kNum] = {false, false, false, false, false, false, false, false, true, true, true, true, true, true, true, true};
+const bool kLp[kNum] = {false, false, false, false, false, false, false, false, true, true, true, true, true, true, true, true, true, true, true, true, true, true};
 
 }  // namespace
 
@@ -62,6 +68,12 @@ static int launch_any(int cfg, const cg_kparams *kp, const cg_group *g, int *occ
     case 13: return launch_lp<uint8_t, 128, 256, 2, 2, 3>(kp, g, occ, st);
     case 14: return launch_lp<uint8_t, 128, 64, 2, 2, 3>(kp, g, occ, st);
     case 15: return launch_lp<uint8_t, 64, 128, 2, 2, 3>(kp, g, occ, st);
+    case 16: return launch_lp<uint8_t, 128, 128, 2, 2, 2>(kp, g, occ, st);
+    case 17: return launch_lp<uint8_t, 256, 64, 4, 1, 2>(kp, g, occ, st);
+    case 18: return launch_lp<uint8_t, 128, 64, 2, 2, 2>(kp, g, occ, st);
+    case 19: return launch_lp<uint8_t, 64, 128, 2, 2, 2>(kp, g, occ, st);
+    case 20: return launch_lp<uint8_t, 256, 128, 2, 2, 2>(kp, g, occ, st);
+    case 21: return launch_lp<uint8_t, 64, 64, 2, 2, 2>(kp, g, occ, st);
     }
     return (int)hipErrorInvalidValue;
 }

@@ -1,10 +1,12 @@
-// lp_kernel.hpp -- pipelined implicit-GEMM forward kernel (NHWC, BF16/FP8).
+// lp_kernel.hpp -- pipelined implicit-GEMM forward kernel (NHWC; FP32, BF16, FP8).
 //
-// The forward NHWC convolution of src/convGemm.c:48-70 for 2-byte and
-// 1-byte operands, where one K step of the general kernel (8 MFMAs per wave)
-// is too short to cover the latency of its gathered loads.  Here:
-//   * every K step moves 128 bytes of each operand row (64 bf16 / 128 fp8
-//     channels of one tap): 16 MFMAs per wave for a 64x64 wave tile;
+// The forward NHWC convolution of src/convGemm.c:48-70 with 16-byte operand
+// chunks, where one K step of the general kernel is too short to cover the
+// latency of its gathered loads, or its register staging costs too many
+// instructions.  Here:
+//   * every K step moves 128 bytes of each operand row (32 fp32 / 64 bf16 /
+//     128 fp8 channels of one tap): for a 64x64 wave tile, 64 / 16 / 8 MFMAs
+//     per wave;
 //   * operands go from HBM straight into LDS (buffer_load_dwordx4 ... lds:
 //     each wave instruction fills 8 rows x 128 B), no register staging, so
 //     the pipeline is NS LDS stages deep (NS-1 steps of loads in flight);
@@ -38,7 +40,7 @@ typedef __attribute__((address_space(3))) void lds_void_t;
 
 template <typename T, int BM, int BN, int WM, int WN, int NS>
 __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &p, const int lin) {
-    static_assert(sizeof(T) == 2 || sizeof(T) == 1, "BF16 or FP8 operands");
+    static_assert(sizeof(T) == 4 || sizeof(T) == 2 || sizeof(T) == 1, "FP32, BF16 or FP8 operands");
     constexpr int ES = sizeof(T);
     constexpr int RB = 128;                 // bytes per row per stage
     constexpr int WTM = BM / WM, WTN = BN / WN;
@@ -123,36 +125,52 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
         else return st3;
     };
 
-    // gather one K step into a stage (LPS vm operations per wave); live ==
-    // false loads zeros (out-of-range offsets).  Issues are never skipped: a
-    // conditional issue makes the compiler drain every load before the next
-    // LDS read.
-    auto issue = [&](auto s, bool live) __attribute__((always_inline)) {
-        char *base = stage_ptr(s);
+    // Gather one K step into a stage: LPS vm operations per wave, A rows
+    // first.  issue_part sends operations [J0, J1) and advance() moves to the
+    // next K step once all of them are out.  live == false loads zeros
+    // (out-of-range offsets).  Issues are never skipped: a conditional issue
+    // makes the compiler drain every load before the next LDS read.
+    auto issue_part = [&](auto s, bool live, auto j0, auto j1) __attribute__((always_inline)) {
+        constexpr int J0 = decltype(j0)::value, J1 = decltype(j1)::value;
+        // The destination is passed as a plain 32-bit LDS address.  Derived
+        // from the stage array, the DMA would carry that array's alias scope
+        // and the compiler would add vmcnt waits of its own before later LDS
+        // reads, on top of the explicit ones below; those waits cut the
+        // loads in flight from NS-1 stages to one.
+        const unsigned base = (unsigned)(uintptr_t)(lds_void_t *)stage_ptr(s);
         const int dh = i_kx * p.vd, dw = i_ky * p.hd;
         const int tapA = (dh * p.W + dw) * CB + i_cb * RB;
         const int lim = CB - i_cb * RB;  // channel bytes left in this block
         const int tapB = i_tap * CB + i_cb * RB;
 #pragma unroll
-        for (int j = 0; j < IA; j++) {
-            const int ih = a_ih[j] + dh, iw = a_iw[j] + dw;
-            const bool ok = live && (unsigned)ih < (unsigned)p.H && (unsigned)iw < (unsigned)p.W && a_cb[j] < lim;
-            const unsigned off = ok ? (unsigned)(a_off[j] + tapA) : OOB;
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rA, (lds_void_t *)(base + (wave * IA + j) * 1024), 16, off, 0,
-                                                     0, 0);
+        for (int jj = J0; jj < J1; jj++) {
+            if (jj < IA) {
+                const int j = jj;
+                const int ih = a_ih[j] + dh, iw = a_iw[j] + dw;
+                const bool ok =
+                    live && (unsigned)ih < (unsigned)p.H && (unsigned)iw < (unsigned)p.W && a_cb[j] < lim;
+                const unsigned off = ok ? (unsigned)(a_off[j] + tapA) : OOB;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rA, (lds_void_t *)(uintptr_t)(base + (wave * IA + j) * 1024), 16, off, 0, 0, 0);
+            } else {
+                const int j = jj - IA;
+                const bool ok = live && b_ok[j] && b_cb[j] < lim;
+                const unsigned off = ok ? (unsigned)(b_off[j] + tapB) : OOB;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rB, (lds_void_t *)(uintptr_t)(base + BM * RB + (wave * IB + j) * 1024), 16, off, 0, 0, 0);
+            }
         }
-#pragma unroll
-        for (int j = 0; j < IB; j++) {
-            const bool ok = live && b_ok[j] && b_cb[j] < lim;
-            const unsigned off = ok ? (unsigned)(b_off[j] + tapB) : OOB;
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rB, (lds_void_t *)(base + BM * RB + (wave * IB + j) * 1024),
-                                                     16, off, 0, 0, 0);
-        }
+    };
+    auto advance = [&]() __attribute__((always_inline)) {
         if (++i_cb == p.kchunks) {
             i_cb = 0;
             ++i_tap;
             if (++i_ky == p.KW) { i_ky = 0; ++i_kx; }
         }
+    };
+    auto issue = [&](auto s, bool live) __attribute__((always_inline)) {
+        issue_part(s, live, std::integral_constant<int, 0>{}, std::integral_constant<int, LPS>{});
+        advance();
     };
 
     // ---- MFMA ----
@@ -168,70 +186,129 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
 #pragma unroll
             for (int v = 0; v < 16; v++) acc[a][b][v] = 0.f;
 
-    // MFMA groups [G0, G1) of a stage (4 groups of 16 bf16 / 2 of 64 fp8)
-    constexpr int NG = ES == 2 ? 4 : 2;
-    auto compute = [&](auto s, auto g0, auto g1) __attribute__((always_inline)) {
-        constexpr int G0 = decltype(g0)::value, G1 = decltype(g1)::value;
+    // A stage holds NG groups of MFMA operands per lane: one 16-byte chunk
+    // (4 fp32, used by 4 MFMAs of K = 2, or 8 bf16 for one MFMA of K = 16)
+    // or two chunks (32 fp8, one MFMA of K = 64).  Lane half h takes chunk
+    // 2g + h (fp8: 4g + 2h and 4g + 2h + 1) of group g for both operands, so
+    // every K element of the group is summed once.  Two register sets
+    // alternate: the LDS reads of group g+1 are in flight while group g
+    // multiplies.
+    constexpr int NG = ES == 1 ? 2 : 4;
+    static_assert(NG % 2 == 0, "the last group of a stage uses register set 1");
+    typedef typename std::conditional<ES == 1, i32x8, i32x4>::type OpV;
+    struct Ops {
+        OpV a[TR], b[TC];
+    };
+    auto ldg = [&](auto s, int g, Ops &r) __attribute__((always_inline)) {
         const char *base = stage_ptr(s);
-        if constexpr (ES == 2) {
+        if constexpr (ES != 1) {
+            const int co = ((2 * g + h) ^ swz) * 16;
 #pragma unroll
-            for (int g = G0; g < G1; g++) {  // 16 bf16 per group
-                const int c = 2 * g + h;
-                const int co = (c ^ swz) * 16;
-                u16x8 ar[TR], bc[TC];
+            for (int a = 0; a < TR; a++) r.a[a] = *(const i32x4 *)(base + rowA + a * 32 * RB + co);
 #pragma unroll
-                for (int a = 0; a < TR; a++) ar[a] = *(const u16x8 *)(base + rowA + a * 32 * RB + co);
-#pragma unroll
-                for (int b = 0; b < TC; b++) bc[b] = *(const u16x8 *)(base + rowB + b * 32 * RB + co);
-#pragma unroll
-                for (int a = 0; a < TR; a++)
-#pragma unroll
-                    for (int b = 0; b < TC; b++)
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            __builtin_bit_cast(bf16x8, ar[a]), __builtin_bit_cast(bf16x8, bc[b]), acc[a][b], 0, 0, 0);
-            }
+            for (int b = 0; b < TC; b++) r.b[b] = *(const i32x4 *)(base + rowB + b * 32 * RB + co);
         } else {
+            const int c0 = 4 * g + 2 * h;
+            const int co0 = (c0 ^ swz) * 16, co1 = ((c0 + 1) ^ swz) * 16;
 #pragma unroll
-            for (int g = G0; g < G1; g++) {  // 64 fp8 per group
-                const int c0 = 4 * g + 2 * h;
-                const int co0 = (c0 ^ swz) * 16, co1 = ((c0 + 1) ^ swz) * 16;
-                i32x8 ar[TR], bc[TC];
+            for (int a = 0; a < TR; a++) {
+                const i32x4 lo = *(const i32x4 *)(base + rowA + a * 32 * RB + co0);
+                const i32x4 hi = *(const i32x4 *)(base + rowA + a * 32 * RB + co1);
+                r.a[a] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+            }
 #pragma unroll
-                for (int a = 0; a < TR; a++) {
-                    const i32x4 lo = *(const i32x4 *)(base + rowA + a * 32 * RB + co0);
-                    const i32x4 hi = *(const i32x4 *)(base + rowA + a * 32 * RB + co1);
-                    ar[a] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-                }
-#pragma unroll
-                for (int b = 0; b < TC; b++) {
-                    const i32x4 lo = *(const i32x4 *)(base + rowB + b * 32 * RB + co0);
-                    const i32x4 hi = *(const i32x4 *)(base + rowB + b * 32 * RB + co1);
-                    bc[b] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-                }
-#pragma unroll
-                for (int a = 0; a < TR; a++)
-#pragma unroll
-                    for (int b = 0; b < TC; b++)
-                        acc[a][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(ar[a], bc[b], acc[a][b], 0, 0, 0,
-                                                                                    127, 0, 127);
+            for (int b = 0; b < TC; b++) {
+                const i32x4 lo = *(const i32x4 *)(base + rowB + b * 32 * RB + co0);
+                const i32x4 hi = *(const i32x4 *)(base + rowB + b * 32 * RB + co1);
+                r.b[b] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
             }
         }
     };
+    auto mma = [&](const Ops &r) __attribute__((always_inline)) {
+        if constexpr (ES == 4) {
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int a = 0; a < TR; a++)
+#pragma unroll
+                    for (int b = 0; b < TC; b++) {
+                        // element j goes through a scalar: a bit_cast applied to
+                        // a vector-element lvalue compiled to element 0 here
+                        const int ea = r.a[a][j], eb = r.b[b][j];
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(__builtin_bit_cast(float, ea),
+                                                                        __builtin_bit_cast(float, eb), acc[a][b], 0,
+                                                                        0, 0);
+                    }
+        } else {
+#pragma unroll
+            for (int a = 0; a < TR; a++)
+#pragma unroll
+                for (int b = 0; b < TC; b++) {
+                    if constexpr (ES == 2)
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, r.a[a]),
+                                                                            __builtin_bit_cast(bf16x8, r.b[b]),
+                                                                            acc[a][b], 0, 0, 0);
+                    else
+                        acc[a][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(r.a[a], r.b[b], acc[a][b], 0,
+                                                                                    0, 0, 127, 0, 127);
+                }
+        }
+    };
 
-    // Step i reads stage i % NS.  Before it: this wave's loads of step i are
-    // complete (the NS-2 later stages may still be in flight), its LDS reads
-    // of step i-1 are complete, then the barrier makes every wave's part
-    // visible and frees stage (i-1) % NS for the loads of step i + NS - 1.
-    auto step = [&](auto s, int i) __attribute__((always_inline)) {
-        constexpr int S = decltype(s)::value;
-        wait_vm<(NS - 2) * LPS>();
+    // The operands of one stage, with the MFMAs shifted by one group: on
+    // entry set 1 holds the previous stage's last group (zeros before the
+    // first stage), on exit it holds this stage's last group.  Its MFMAs
+    // cover the LDS latency of this stage's first reads.  Each of the NG
+    // regions reads one group, multiplies the one before it and, with
+    // issue_next, sends a share of the loads of step i + NS - 1, whose
+    // address arithmetic fills the MFMA shadow.  The scheduling fences keep
+    // the compiler from regrouping each read next to its MFMAs.
+    Ops r0, r1;
+#pragma unroll
+    for (int a = 0; a < TR; a++) r0.a[a] = OpV{};
+#pragma unroll
+    for (int b = 0; b < TC; b++) r0.b[b] = OpV{};
+    r1 = r0;
+    constexpr int PER = (LPS + NG - 1) / NG;  // loads per region
+    auto stage_mma = [&](auto s, auto issue_next, int i) __attribute__((always_inline)) {
+        constexpr bool ISS = decltype(issue_next)::value;
+        constexpr int SN = (decltype(s)::value + NS - 1) % NS;
+        const bool live = i + NS - 1 < nsteps;
+        auto region = [&](auto g) __attribute__((always_inline)) {
+            constexpr int G = decltype(g)::value;
+            __builtin_amdgcn_sched_barrier(0);
+            if constexpr (G % 2 == 0) {
+                ldg(s, G, r0);
+                mma(r1);
+            } else {
+                ldg(s, G, r1);
+                mma(r0);
+            }
+            if constexpr (ISS && G * PER < LPS)
+                issue_part(std::integral_constant<int, SN>{}, live, std::integral_constant<int, G * PER>{},
+                           std::integral_constant<int, (G * PER + PER < LPS ? G * PER + PER : LPS)>{});
+        };
+        region(std::integral_constant<int, 0>{});
+        region(std::integral_constant<int, 1>{});
+        if constexpr (NG > 2) {
+            region(std::integral_constant<int, (NG > 2 ? 2 : 0)>{});
+            region(std::integral_constant<int, (NG > 2 ? 3 : 1)>{});
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if constexpr (ISS) advance();
+    };
+    // Before step i: this wave's loads of step i are complete (the NS-2
+    // later stages may still be in flight) and so are its LDS reads of step
+    // i-1; the barrier then makes every wave's part of stage i visible and
+    // frees stage (i-1) % NS for the loads of step i + NS - 1.
+    auto sync = [&](auto pending) __attribute__((always_inline)) {
+        wait_vm<decltype(pending)::value>();
         __builtin_amdgcn_s_waitcnt((0xF) | (0x7 << 4) | (0x0 << 8) | (0x3 << 14));  // lgkmcnt(0)
         __builtin_amdgcn_s_barrier();
-        // the first MFMA group goes ahead of the loads' address arithmetic
-        compute(s, std::integral_constant<int, 0>{}, std::integral_constant<int, 1>{});
-        issue(std::integral_constant<int, (S + NS - 1) % NS>{}, i + NS - 1 < nsteps);
-        compute(s, std::integral_constant<int, 1>{}, std::integral_constant<int, NG>{});
     };
+    constexpr auto kInFlight = std::integral_constant<int, (NS - 2) * LPS>{};
+    constexpr auto kYes = std::true_type{};
+    constexpr auto kNo = std::false_type{};
 
     if (nsteps > 0) {
         issue(std::integral_constant<int, 0>{}, true);
@@ -239,25 +316,30 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
         if constexpr (NS > 3) issue(std::integral_constant<int, (NS > 3 ? 2 : 0)>{}, 2 < nsteps);
         int i = 0;
         for (; i + NS <= nsteps; i += NS) {
-            step(std::integral_constant<int, 0>{}, i);
-            step(std::integral_constant<int, 1>{}, i + 1);
-            if constexpr (NS > 2) step(std::integral_constant<int, (NS > 2 ? 2 : 0)>{}, i + 2);
-            if constexpr (NS > 3) step(std::integral_constant<int, (NS > 3 ? 3 : 0)>{}, i + 3);
+            sync(kInFlight);
+            stage_mma(std::integral_constant<int, 0>{}, kYes, i);
+            sync(kInFlight);
+            stage_mma(std::integral_constant<int, 1>{}, kYes, i + 1);
+            if constexpr (NS > 2) {
+                sync(kInFlight);
+                stage_mma(std::integral_constant<int, (NS > 2 ? 2 : 0)>{}, kYes, i + 2);
+            }
+            if constexpr (NS > 3) {
+                sync(kInFlight);
+                stage_mma(std::integral_constant<int, (NS > 3 ? 3 : 0)>{}, kYes, i + 3);
+            }
         }
         // tail: the last nsteps % NS steps sit in stages 0.. and are all
         // issued; nothing is overwritten any more
         const int rem = nsteps - i;
         if (rem > 0) {
-            wait_vm<0>();
-            __builtin_amdgcn_s_waitcnt((0xF) | (0x7 << 4) | (0x0 << 8) | (0x3 << 14));
-            __builtin_amdgcn_s_barrier();
-            constexpr auto g0 = std::integral_constant<int, 0>{};
-            constexpr auto gN = std::integral_constant<int, NG>{};
-            compute(std::integral_constant<int, 0>{}, g0, gN);
-            if (rem > 1) compute(std::integral_constant<int, 1>{}, g0, gN);
+            sync(std::integral_constant<int, 0>{});
+            stage_mma(std::integral_constant<int, 0>{}, kNo, 0);
+            if (rem > 1) stage_mma(std::integral_constant<int, 1>{}, kNo, 0);
             if constexpr (NS > 3)
-                if (rem > 2) compute(std::integral_constant<int, (NS > 3 ? 2 : 0)>{}, g0, gN);
+                if (rem > 2) stage_mma(std::integral_constant<int, (NS > 3 ? 2 : 0)>{}, kNo, 0);
         }
+        mma(r1);
     }
 
     // ---- write-back (FWD NHWC: rows = output pixels, cols = kn) ----

@@ -170,7 +170,7 @@ def test_every_lowp_configuration(dtype):
                 assert rel_err(y, ref)[0] <= (2e-5 if dtype == "bf16" else FP8_TOL), (layout, cfg, splits)
 
 
-# Shapes for the pipelined forward kernel (128-byte K steps, 3-4 LDS stages):
+# Shapes for the pipelined forward kernel (128-byte K steps, 2-4 LDS stages):
 # a single K step, every tail length, ragged channel blocks, padding, stride,
 # dilation, M and N tails.
 LP_SHAPES = [
@@ -183,10 +183,10 @@ LP_SHAPES = [
 ]
 
 
-@pytest.mark.parametrize("dtype", ["bf16", "fp8"])
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "fp8"])
 @pytest.mark.parametrize("si", range(len(LP_SHAPES)))
 def test_pipelined_forward_configurations(dtype, si):
-    elem = cg.BF16 if dtype == "bf16" else cg.FP8_E4M3
+    elem = {"f32": cg.F32, "bf16": cg.BF16, "fp8": cg.FP8_E4M3}[dtype]
     lp = [i for i, t in enumerate(cg.tile_configs(elem)) if t[5] > 0]
     assert lp, "no pipelined configurations"
     s = LP_SHAPES[si]
@@ -195,12 +195,15 @@ def test_pipelined_forward_configurations(dtype, si):
     rng = np.random.default_rng(50 + si)
     X = rng.standard_normal(s.x_shape(NHWC)).astype(np.float32)
     W = (rng.standard_normal(s.w_shape(NHWC)) * 0.1).astype(np.float32)
-    if dtype == "bf16":
+    if dtype == "f32":
+        Xd, Xq, Wd, Wq, scales = torch.from_numpy(X).to(DEV), X, torch.from_numpy(W).to(DEV), W, None
+    elif dtype == "bf16":
         (Xd, Xq), (Wd, Wq), scales = q_bf16(X), q_bf16(W), None
     else:
         Xd, Xq, sx = q_fp8(X)
         Wd, Wq, sw = q_fp8(W)
         scales = (sw, sx)
+    tol = {"f32": 1e-5, "bf16": 2e-5, "fp8": FP8_TOL}[dtype]
     epi = dict(bias=(0.1 * rng.standard_normal(s.kn)).astype(np.float32), relu=True)
     ref = oracle.fwd(NHWC, s, Wq, Xq, **epi)
     geo = (s.b, s.h, s.w, s.c, s.kn, s.kh, s.kw, s.vpad, s.hpad, s.vstr, s.hstr, s.vdil, s.hdil)
@@ -218,4 +221,4 @@ def test_pipelined_forward_configurations(dtype, si):
             finally:
                 cg.force_config(-1, 0)
             y = Y.cpu().numpy()
-            assert rel_err(y, ref)[0] <= (2e-5 if dtype == "bf16" else FP8_TOL), (cfg, splits)
+            assert rel_err(y, ref)[0] <= tol, (cfg, splits)
