@@ -196,6 +196,8 @@ def lib() -> ctypes.CDLL:
     L.cg_num_tilecfgs.restype = ci
     L.cg_get_tilecfg.argtypes = [ci, ci, ctypes.POINTER(ci * 6)]
     L.cg_get_tilecfg.restype = ci
+    L.cg_tilecfg_supports.argtypes = [ci, ci, ci, ci]
+    L.cg_tilecfg_supports.restype = ci
     L.cg_force_config.argtypes = [ci, ci]
     L.cg_force_config.restype = None
     L.cg_last_plan.argtypes = [ctypes.POINTER(ci)] * 3
@@ -604,6 +606,13 @@ def tile_configs(dtype=F32):
         if L.cg_get_tilecfg(int(dtype), c, ctypes.byref(v)) == 0:
             out.append(tuple(v))
     return out
+
+
+def tile_supports(dtype, cfg, op, layout, vec=True) -> bool:
+    """Is tile configuration `cfg` instantiated for (op, layout), with 16-byte operand loads if `vec`?"""
+    opkind = {(FWD, NHWC): 0, (FWD, NCHW): 1, (WGRAD, NHWC): 2, (WGRAD, NCHW): 3, (DGRAD, NHWC): 4,
+              (DGRAD, NCHW): 5}[(op, layout)]
+    return bool(lib().cg_tilecfg_supports(int(dtype), int(cfg), opkind, int(bool(vec))))
 
 
 def packed_weights_size(desc: Desc, dtype=F32) -> int:

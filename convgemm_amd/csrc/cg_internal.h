@@ -65,7 +65,7 @@ typedef struct {
 typedef struct {
     int BM, BN, BK;
     int WM, WN;      /* waves along M and N (WM*WN = 4) */
-    int lp;          /* > 0: the pipelined forward-NHWC kernel with lp LDS stages of 128-byte rows */
+    int lp;          /* > 0: a pipelined forward kernel (NHWC; NCHW for FP32) with lp LDS stages */
 } cg_tilecfg;
 
 int cg_num_tilecfgs(int elem);

@@ -2,6 +2,7 @@
 // its tile-configuration table (BM, BN, BK, waves along M, waves along N, pipelined).
 #include "igemm_kernel.hpp"
 #include "lp_kernel.hpp"
+#include "lp_nchw_kernel.hpp"
 
 using namespace cg;
 
@@ -30,14 +31,26 @@ const cg_tilecfg kCfg[] = {
     {64, 64, 32, 2, 2, 4},  // 19 pipelined, 4 LDS stages
     {256, 64, 32, 4, 1, 2},  // 20 pipelined, 2 LDS stages
     {128, 64, 32, 2, 2, 2},  // 21 pipelined, 2 LDS stages
+    {128, 128, 32, 2, 2, 3},  // 22 pipelined NCHW, 3 LDS stages
+    {128, 128, 32, 2, 2, 2},  // 23 pipelined NCHW, 2 LDS stages
+    {128, 64, 32, 2, 2, 3},  // 24 pipelined NCHW, 3 LDS stages
+    {64, 128, 32, 2, 2, 3},  // 25 pipelined NCHW, 3 LDS stages
+    {64, 64, 32, 2, 2, 4},  // 26 pipelined NCHW, 4 LDS stages
+    {256, 64, 32, 4, 1, 2},  // 27 pipelined NCHW, 2 LDS stages
 };
 constexpr int kNum = sizeof(kCfg) / sizeof(kCfg[0]);
 // configurations that also have element-load (non-16-byte) instantiations
 const bool kNoVec[kNum] = {true, false, false, true, false, false, true, false, false, false, false, false,
-                           false, false, false, false, false, false, false, false, false, false};
+                           false, false, false, false, false, false, false, false, false, false,
+                           false, false, false, false, false, false};
 // configurations served by the pipelined forward kernel (lp_kernel.hpp)
 const bool kLp[kNum] = {false, false, false, false, false, false, false, false, false, false, false, false,
-                        true, true, true, true, true, true, true, true, true, true};
+                        true, true, true, true, true, true, true, true, true, true,
+                        false, false, false, false, false, false};
+// configurations served by the pipelined forward NCHW kernel (lp_nchw_kernel.hpp)
+const bool kLpn[kNum] = {false, false, false, false, false, false, false, false, false, false, false, false,
+                         false, false, false, false, false, false, false, false, false, false,
+                         true, true, true, true, true, true};
 
 }  // namespace
 
@@ -49,7 +62,7 @@ extern "C" int cg_f32_get_cfg(int cfg, cg_tilecfg *out) {
     return 0;
 }
 
-extern "C" int cg_f32_supports(int cfg, int op, int vec) { return supports<float>(kCfg, kNoVec, kNum, cfg, op, vec, kLp); }
+extern "C" int cg_f32_supports(int cfg, int op, int vec) { return supports<float>(kCfg, kNoVec, kNum, cfg, op, vec, kLp, kLpn); }
 
 static int launch_any(int cfg, const cg_kparams *kp, const cg_group *g, int *occ, void *stream) {
     hipStream_t st = (hipStream_t)stream;
@@ -76,6 +89,12 @@ static int launch_any(int cfg, const cg_kparams *kp, const cg_group *g, int *occ
     case 19: return launch_lp<float, 64, 64, 2, 2, 4>(kp, g, occ, st);
     case 20: return launch_lp<float, 256, 64, 4, 1, 2>(kp, g, occ, st);
     case 21: return launch_lp<float, 128, 64, 2, 2, 2>(kp, g, occ, st);
+    case 22: return launch_lpn<128, 128, 2, 2, 3>(kp, g, occ, st);
+    case 23: return launch_lpn<128, 128, 2, 2, 2>(kp, g, occ, st);
+    case 24: return launch_lpn<128, 64, 2, 2, 3>(kp, g, occ, st);
+    case 25: return launch_lpn<64, 128, 2, 2, 3>(kp, g, occ, st);
+    case 26: return launch_lpn<64, 64, 2, 2, 4>(kp, g, occ, st);
+    case 27: return launch_lpn<256, 64, 4, 1, 2>(kp, g, occ, st);
     }
     return (int)hipErrorInvalidValue;
 }

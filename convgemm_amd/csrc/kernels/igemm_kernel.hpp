@@ -684,9 +684,10 @@ inline int launch_cfg(const cg_kparams *kp, const cg_group *g, int *occ, hipStre
 // kernel (lp_kernel.hpp): forward NHWC with 16-byte channel chunks only.
 template <typename T>
 inline int supports(const cg_tilecfg *table, const bool *novec, int n, int cfg, int op, int vec,
-                    const bool *lp = nullptr) {
+                    const bool *lp = nullptr, const bool *lpn = nullptr) {
     if (cfg < 0 || cfg >= n || op < 0 || op >= CG_NUM_OPKINDS) return 0;
     if (lp && lp[cfg]) return op == CG_FWD_NHWC && vec;
+    if (lpn && lpn[cfg]) return op == CG_FWD_NCHW && vec;
     const bool en[6] = {op_enabled<T>(0), op_enabled<T>(1), op_enabled<T>(2),
                         op_enabled<T>(3), op_enabled<T>(4), op_enabled<T>(5)};
     const bool vr[6] = {vec_relevant<T>(0), vec_relevant<T>(1), vec_relevant<T>(2),

@@ -187,7 +187,7 @@ LP_SHAPES = [
 @pytest.mark.parametrize("si", range(len(LP_SHAPES)))
 def test_pipelined_forward_configurations(dtype, si):
     elem = {"f32": cg.F32, "bf16": cg.BF16, "fp8": cg.FP8_E4M3}[dtype]
-    lp = [i for i, t in enumerate(cg.tile_configs(elem)) if t[5] > 0]
+    lp = [i for i, t in enumerate(cg.tile_configs(elem)) if t[5] > 0 and cg.tile_supports(elem, i, cg.FWD, NHWC)]
     assert lp, "no pipelined configurations"
     s = LP_SHAPES[si]
     if dtype == "fp8" and s.c % 16:
@@ -222,3 +222,35 @@ def test_pipelined_forward_configurations(dtype, si):
                 cg.force_config(-1, 0)
             y = Y.cpu().numpy()
             assert rel_err(y, ref)[0] <= tol, (cfg, splits)
+
+
+@pytest.mark.parametrize("si", range(len(LP_SHAPES)))
+def test_pipelined_forward_nchw_fp32(si):
+    """The pipelined NCHW forward kernel (FP32, 4-byte activation loads into a
+    K-major LDS tile) on every tail, padding, stride and dilation case, with
+    the fused epilogue and split-K, against the FP64 oracle."""
+    lpn = [i for i, t in enumerate(cg.tile_configs(cg.F32))
+           if t[5] > 0 and cg.tile_supports(cg.F32, i, cg.FWD, NCHW)]
+    assert lpn, "no pipelined NCHW configurations"
+    s = LP_SHAPES[si]
+    rng = np.random.default_rng(70 + si)
+    X = rng.standard_normal(s.x_shape(NCHW)).astype(np.float32)
+    W = (rng.standard_normal(s.w_shape(NCHW)) * 0.1).astype(np.float32)
+    epi = dict(bias=(0.1 * rng.standard_normal(s.kn)).astype(np.float32), relu=True)
+    ref = oracle.fwd(NCHW, s, W, X, **epi)
+    geo = (s.b, s.h, s.w, s.c, s.kn, s.kh, s.kw, s.vpad, s.hpad, s.vstr, s.hstr, s.vdil, s.hdil)
+    d = cg.make_desc(cg.FWD, NCHW, *geo)
+    Xd, Wd = torch.from_numpy(X).to(DEV), torch.from_numpy(W).to(DEV)
+    bias = torch.from_numpy(epi["bias"]).to(DEV)
+    for cfg in lpn:
+        for splits in (1, 2, 3):
+            cg.force_config(cfg, splits)
+            try:
+                Y = torch.full(s.y_shape(NCHW), float("nan"), device=DEV)
+                print(f"lp nchw f32 shape {si} cfg {cfg} splits {splits}", flush=True)
+                cg.conv_ex(d, Wd, Xd, Y, bias=bias, relu=True)
+                torch.cuda.synchronize()
+                assert cg.last_plan()[0] == cfg
+            finally:
+                cg.force_config(-1, 0)
+            assert rel_err(Y.cpu().numpy(), ref)[0] <= 1e-5, (cfg, splits)
