@@ -9,6 +9,11 @@ with the fused bias + batch-norm + ReLU epilogue: 10 convolutions, 57.6 GFLOP
 Inputs are synthetic (X ~ N(0,1), He-scaled weights, epilogue vectors as in
 the workload description) and resident in HBM before timing starts.
 
+Plans: by default the tuning database shipped in convgemm_amd/data (the
+autotuner's timings of these ten problems on an MI355X, the plans the
+committed profiles were measured with) is loaded read-only; problems it
+lacks, or every problem with --fresh-tune, are autotuned at setup (untimed).
+
 Prints one JSON line:  value = TFLOP/s over the timed steps, the roofline
 of the dominant kernel measured with HIP events on the launch stream, the
 FP64 error of every layer, and a CPU baseline (the oracle's FP32
@@ -27,6 +32,7 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+SHIPPED_TUNE_DB = os.path.join(ROOT, "convgemm_amd", "data", "mi355x_fp32_sweep_b32.tunedb")
 sys.path.insert(0, ROOT)
 
 METRIC = "conv TFLOP/s (% of roofline), BF16/FP8 vs FP32 plain at stated FP64 error"
@@ -334,6 +340,36 @@ def copy_probe(torch, mib, steps, warmup):
     return 0
 
 
+def vendor_gemm(torch, reps):
+    """hipBLASLt (torch.matmul) on the GEMM each forward layer is at batch 64
+    (M = b*ho*wo, K = c*kh*kw, N = kn, A materialized, no gather) and on a
+    square 8192^3 problem: what the vendor library reaches on these shapes,
+    next to which the implicit-GEMM convolution's TF/s can be read."""
+    shapes = [(name, 64 * out_hw(h, pad, kh, st) * out_hw(w, pad, kw, st), c * kh * kw, kn)
+              for name, (b, h, w, c, kn, kh, kw, pad, st) in LAYERS] + [("square", 8192, 8192, 8192)]
+    out = {}
+    for dt, tag in ((torch.bfloat16, "bf16"), (torch.float32, "f32")):
+        for name, M, K, N in shapes:
+            a = torch.randn(M, K, device="cuda").to(dt)
+            b = torch.randn(K, N, device="cuda").to(dt)
+            c = torch.empty(M, N, device="cuda", dtype=dt)
+            for _ in range(3):
+                torch.matmul(a, b, out=c)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                torch.matmul(a, b, out=c)
+            e1.record()
+            torch.cuda.synchronize()
+            us = e0.elapsed_time(e1) * 1e3 / reps
+            out[f"{name}_{tag}"] = {"M": M, "K": K, "N": N, "us": round(us, 2),
+                                    "tflops": round(2.0 * M * K * N / us / 1e6, 1)}
+            del a, b, c
+    print(json.dumps({"vendor_gemm": "hipBLASLt via torch.matmul", "layers_at_batch": 64, "reps": reps,
+                      "results": out}), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -347,6 +383,10 @@ def main(argv=None):
     ap.add_argument("--no-backward", action="store_true", help="skip the backward (wgrad/dgrad) sweep")
     ap.add_argument("--tune-db", default="", help="tuning database: loaded if present (layers found there are "
                     "not re-tuned), written back after tuning")
+    ap.add_argument("--fresh-tune", action="store_true",
+                    help="do not load the shipped tuning database: autotune every layer at setup")
+    ap.add_argument("--vendor-gemm", action="store_true",
+                    help="only time hipBLASLt (torch.matmul) on the layers' GEMM shapes, for calibration")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB per step (calibrates HBM byte counters)")
     ap.add_argument("--only", default="", help="comma-separated layer names (e.g. L2_3x3_64to64_56_NHWC) "
@@ -365,8 +405,15 @@ def main(argv=None):
     stream = torch.cuda.current_stream()
     if a.copy_probe:
         return copy_probe(torch, a.copy_probe, a.steps, a.warmup)
+    if a.vendor_gemm:
+        return vendor_gemm(torch, a.steps)
+    plans = "autotuned at setup"
     if a.tune_db and os.path.exists(a.tune_db):
         cg.tune_load(a.tune_db)
+        plans = f"tuning database {os.path.basename(a.tune_db)}, then autotuned"
+    elif not a.tune_db and not a.fresh_tune and not a.no_tune and os.path.exists(SHIPPED_TUNE_DB):
+        cg.tune_load(SHIPPED_TUNE_DB)
+        plans = "shipped tuning database (MI355X autotuner timings)"
     convs = []
     for layout in (0, 1):
         for li, (name, spec) in enumerate(LAYERS):
@@ -491,6 +538,7 @@ def main(argv=None):
         "config": {"workload": "resnet50_layer_sweep_fwd_b32_nhwc_nchw_bias_bn_relu",
                    "layers": [n for n, _ in LAYERS], "layouts": ["NHWC", "NCHW"], "global_batch": 32,
                    "gflop_per_step": round(step_flops / 1e9, 3)},
+        "plans": plans if not a.no_tune else "analytic planner",
         "roofline": {"bound": "mfma", "kernel": dom["name"], "kernel_us": round(kernel_ms * 1e3, 2),
                      "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
