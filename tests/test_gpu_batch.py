@@ -181,6 +181,51 @@ def test_uneven_forward_batch(dtype, flags):
     b.close()
 
 
+def pipelined_cases(dtype, layout, n, seed):
+    """Forward problems the pipelined kernels serve: 16-byte channel chunks,
+    padding, stride 2, ragged M/N/K tails, with and without the epilogue."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        k = int(rng.choice([1, 3]))
+        s = int(rng.choice([1, 2]))
+        h = int(rng.choice([7, 12, 20]))
+        c = int(rng.choice(range(16, 161, 16)))
+        kn = int(rng.choice(range(16, 161, 16)))
+        out.append(Case(D(cg.FWD, layout, int(rng.choice([1, 2, 3])), h, h, c, kn, k, s=s), dtype, seed + i,
+                        alpha=0.75 if i % 4 == 1 else 1.0, beta=1.0 if i % 4 == 3 else 0.0,
+                        epi=BN_RELU if i % 2 == 0 else None))
+    return out
+
+
+@pytest.mark.parametrize("dtype,layout", [(cg.F32, cg.NHWC), (cg.F32, cg.NCHW), (cg.BF16, cg.NHWC),
+                                          (cg.FP8_E4M3, cg.NHWC)])
+@pytest.mark.parametrize("flags", [cg.BATCH_GRAPH, cg.BATCH_GRAPH | cg.BATCH_PERSISTENT])
+def test_pipelined_groups(dtype, layout, flags):
+    """Grouped (and persistent) launches of the pipelined kernels: every
+    pipelined configuration forced in turn, with and without split-K."""
+    cfgs = [i for i, t in enumerate(cg.tile_configs(dtype)) if t[5] > 0 and cg.tile_supports(dtype, i, cg.FWD, layout)]
+    assert cfgs
+    for cfg in cfgs:
+        for splits in (1, 2):
+            print(f"pipelined group dtype {dtype} layout {layout} cfg {cfg} splits {splits}", flush=True)
+            cg.force_config(cfg, splits)
+            try:
+                cases = pipelined_cases(dtype, layout, 6, 400 + 10 * cfg + splits)
+                refs = [c.reference() for c in cases]
+                assert cg.last_plan()[0] == cfg
+                b = cg.Batch([c.problem() for c in cases], dtype, flags)
+            finally:
+                cg.force_config(-1, 0)
+            for rep in range(2):
+                for c in cases:
+                    c.reset()
+                b.run()
+                torch.cuda.synchronize()
+                check(cases, refs, f"cfg={cfg} splits={splits} replay={rep}")
+            b.close()
+
+
 def test_bf16_backward_batch():
     F, W, G = cg.FWD, cg.WGRAD, cg.DGRAD
     cases = [Case(D(W, cg.NHWC, 2, 10, 10, 32, 48, 3), cg.BF16, 21),
