@@ -49,8 +49,8 @@ typedef struct {
     int vec;         /* reduction channels are a multiple of the 16-byte chunk (see cg_vec_ok) */
     long long P;     /* B*Ho*Wo */
     const void *A;   /* FWD/WGRAD: X;  DGRAD: dY */
-    const void *Bm;  /* FWD F32 / DGRAD: weights in [c][kh][kw][kn] order;
-                        FWD BF16/FP8: weights in [kn][kh][kw][c] order; WGRAD: dY */
+    const void *Bm;  /* FWD: weights in [kn][kh][kw][c] order (repacked, or NCHW 1x1 as given);
+                        DGRAD: weights in [c][kh][kw][kn] order; WGRAD: dY */
     long long a_bytes, b_bytes; /* extents of A and Bm (each < 2 GiB) */
     float *out;
     float *ws;       /* split-K partials: [splits][classes][M][N] */

@@ -148,8 +148,9 @@ convgemm_status_t convgemm_conv_ex(const convgemm_desc_t *desc, float alpha, flo
 
 /* Weight prepacking for repeated calls with the same weights (no counterpart
  * in the reference, which packs on every call).  Some operations read the
- * weights in another order than the caller's (BF16/FP8/FP32 forward:
- * [kn][kh][kw][c]; NCHW backward data: [c][kh][kw][kn]); every call of
+ * weights in another order than the caller's (forward: [kn][kh][kw][c],
+ * except NCHW 1x1 kernels, already in that order; NCHW backward data:
+ * [c][kh][kw][kn]); every call of
  * convgemm_conv_ex then starts with a repack kernel.  convgemm_pack_weights
  * does that repack once into `packed` (convgemm_packed_weights_size bytes;
  * 0 means the operation reads the weights as given), and convgemm_conv_packed

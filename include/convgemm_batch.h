@@ -41,8 +41,8 @@ enum {
      * resident at once, and each walks the group's tiles with a fixed stride */
     CONVGEMM_BATCH_PERSISTENT = 2,
     /* repack the weights once, at creation: later changes to the weights of
-     * problems that need a repack (BF16/FP8 forward, FP32 NCHW forward and
-     * backward data) are not seen by runs */
+     * problems that need a repack (forward except NCHW 1x1, NCHW backward
+     * data) are not seen by runs */
     CONVGEMM_BATCH_STATIC_WEIGHTS = 4
 };
 
