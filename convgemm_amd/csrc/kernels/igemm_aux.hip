@@ -121,6 +121,23 @@ __global__ void strided_transpose_kernel(const E *__restrict__ src, E *__restric
     }
 }
 
+// [d0][d1][d2] -> [d0][d2][d1] for a short d2 (a kernel's taps): one thread per
+// output element, d1 fastest, so the stores coalesce and the loads, d2
+// elements apart, share their lines through L2.  A 32x32 tile would leave
+// most of its threads idle on d2 = 9.
+template <typename E>
+__global__ void permute_short_kernel(const E *__restrict__ src, E *__restrict__ dst, int d1, int d2, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long plane = (long long)d1 * d2;
+    for (; i < n; i += stride) {
+        const long long a = i / plane;
+        const int rem = (int)(i - a * plane);
+        const int t = rem / d1, c = rem - t * d1;  // output [a][t][c]
+        dst[i] = src[a * plane + (long long)c * d2 + t];
+    }
+}
+
 // ---- FP32 -> BF16 (round to nearest even) ----
 __device__ inline uint16_t f32_to_bf16_rne(float f) {
     uint32_t u = __builtin_bit_cast(uint32_t, f);
@@ -352,6 +369,22 @@ extern "C" int cg_launch_permute3(const void *src, void *dst, int d0, int d1, in
         R = d0; Cc = d2; batch = d1;
         sz = d2; sr = (long long)d1 * d2; tz = d0; tc = (long long)d1 * d0;
     } else {          // [d0][d1][d2] -> [d0][d2][d1]: batch over d0
+        if (d2 <= 16) {
+            const long long n = (long long)d0 * d1 * d2;
+            long long blocks = (n + 255) / 256;
+            if (blocks > 8192) blocks = 8192;
+            hipStream_t st = (hipStream_t)stream;
+            switch (esize) {
+            case 4: hipLaunchKernelGGL(permute_short_kernel<float>, dim3((unsigned)blocks), dim3(256), 0, st,
+                                       (const float *)src, (float *)dst, d1, d2, n); break;
+            case 2: hipLaunchKernelGGL(permute_short_kernel<uint16_t>, dim3((unsigned)blocks), dim3(256), 0, st,
+                                       (const uint16_t *)src, (uint16_t *)dst, d1, d2, n); break;
+            case 1: hipLaunchKernelGGL(permute_short_kernel<uint8_t>, dim3((unsigned)blocks), dim3(256), 0, st,
+                                       (const uint8_t *)src, (uint8_t *)dst, d1, d2, n); break;
+            default: return (int)hipErrorInvalidValue;
+            }
+            return (int)hipGetLastError();
+        }
         R = d1; Cc = d2; batch = d0;
         sz = (long long)d1 * d2; sr = d2; tz = (long long)d2 * d1; tc = d1;
     }

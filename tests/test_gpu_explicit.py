@@ -216,10 +216,13 @@ def test_explicit_equals_implicit(si):
 
 
 @pytest.mark.parametrize("esize", [4, 2, 1])
-def test_layout_converters(esize):
+@pytest.mark.parametrize("dims", [(2, 5, 7, 3, 3, 2, 4), (3, 37, 3, 3, 1, 1, 6), (1, 300, 4, 4, 3, 3, 2)])
+def test_layout_converters(esize, dims):
+    """NCHW<->NHWC and weight-order converters; the second and third shapes put
+    16 or fewer elements on the inner axis (the short-permute kernel)."""
     dt = {4: torch.float32, 2: torch.int16, 1: torch.uint8}[esize]
     g = torch.Generator().manual_seed(esize)
-    b, c, h, w, kh, kw, kn = 2, 5, 7, 3, 3, 2, 4
+    b, c, h, w, kh, kw, kn = dims
     x = torch.randint(0, 100, (b, c, h, w), generator=g).to(dt).to(DEV)
     y = torch.empty(b, h, w, c, dtype=dt, device=DEV)
     cg.nchw_to_nhwc(x, y, b, c, h, w, esize)
