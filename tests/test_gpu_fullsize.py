@@ -85,3 +85,52 @@ def test_fullsize_layer(name, layout):
     ref = sampled.dgrad_sampled(layout, s, Wh, dYh, xs)
     got = sampled.nhwc_index(layout, dX.cpu().numpy(), xs)
     assert np.max(np.abs(got - ref)) <= TOL * np.max(np.abs(ref))
+
+
+def _gather(layout, Y, ys):
+    """Y (on the GPU) at NHWC-ordered (b, x, y, k) samples, copied back."""
+    i = [torch.as_tensor(ys[:, j], device=Y.device) for j in range(4)]
+    v = Y[i[0], i[1], i[2], i[3]] if layout == NHWC else Y[i[0], i[3], i[1], i[2]]
+    return v.cpu().numpy().astype(np.float64)
+
+
+@pytest.mark.parametrize("layout", [NHWC, NCHW])
+@pytest.mark.parametrize("name", list(LAYERS))
+def test_fullsize_pipelined_forward(name, layout):
+    """Every pipelined FP32 forward configuration of this layout, forced on the
+    full-size layer with the fused epilogue (split-K 1 and 3): 256 sampled
+    outputs against the FP64 oracle."""
+    s = LAYERS[name]
+    cfgs = [i for i, t in enumerate(cg.tile_configs(cg.F32))
+            if t[5] > 0 and cg.tile_supports(cg.F32, i, cg.FWD, layout)]
+    assert cfgs
+    g = torch.Generator(device="cuda").manual_seed(sum(map(ord, name)) * 3 + layout)
+    X = torch.randn(s.x_shape(layout), device="cuda", generator=g)
+    W = torch.randn(s.w_shape(layout), device="cuda", generator=g) * (2.0 / s.K) ** 0.5
+    kn = s.kn
+    bias = 0.1 * torch.randn(kn, device="cuda", generator=g)
+    mean = 0.1 * torch.randn(kn, device="cuda", generator=g)
+    inv_std = 1.0 / torch.sqrt(0.5 + 1.5 * torch.rand(kn, device="cuda", generator=g) + 1e-5)
+    gamma = 0.5 + torch.rand(kn, device="cuda", generator=g)
+    bnb = 0.1 * torch.randn(kn, device="cuda", generator=g)
+    rng = np.random.default_rng(12)
+    ys = np.stack([rng.integers(0, n, 256) for n in (s.b, s.ho, s.wo, s.kn)], 1)
+    epi = dict(bias=bias.cpu().numpy(), mean=mean.cpu().numpy(), inv_std=inv_std.cpu().numpy(),
+               gamma=gamma.cpu().numpy(), bnbeta=bnb.cpu().numpy(), relu=True)
+    ref = sampled.fwd_sampled(layout, s, W.cpu().numpy(), X.cpu().numpy(), ys, **epi)
+    d = cg.make_desc(cg.FWD, layout, s.b, s.h, s.w, s.c, s.kn, s.kh, s.kw, s.vpad, s.hpad, s.vstr, s.hstr,
+                     s.vdil, s.hdil)
+    Y = torch.empty(s.y_shape(layout), device="cuda")
+    for cfg in cfgs:
+        for splits in (1, 3):
+            print(f"full-size pipelined {name} layout={layout} cfg {cfg} splits {splits}", flush=True)
+            Y.fill_(float("nan"))
+            cg.force_config(cfg, splits)
+            try:
+                cg.conv_ex(d, W, X, Y, bias=bias, mean=mean, inv_std=inv_std, gamma=gamma, bnbeta=bnb, relu=True)
+                torch.cuda.synchronize()
+                assert cg.last_plan()[0] == cfg
+            finally:
+                cg.force_config(-1, 0)
+            got = _gather(layout, Y, ys)
+            assert np.max(np.abs(got - ref)) <= TOL * max(np.max(np.abs(ref)), 1.0), (cfg, splits)
