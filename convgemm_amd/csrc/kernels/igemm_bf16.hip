@@ -1,5 +1,5 @@
 // igemm_bf16.hip -- BF16 (v_mfma_f32_32x32x16_bf16) instantiations of the implicit-GEMM kernel and
-// its tile-configuration table (BM, BN, BK, waves along M, waves along N, pipelined).
+// its tile-configuration table (BM, BN, BK, waves along M, waves along N, pipelined stages).
 #include "igemm_kernel.hpp"
 #include "lp_kernel.hpp"
 
@@ -31,12 +31,19 @@ const cg_tilecfg kCfg[] = {
     {64, 128, 64, 2, 2, 2},  // 20 pipelined, 2 LDS stages
     {256, 128, 64, 2, 2, 2},  // 21 pipelined, 2 LDS stages
     {64, 64, 64, 2, 2, 2},  // 22 pipelined, 2 LDS stages
+    {256, 128, 64, 4, 2, 2},  // 23 pipelined, 8 waves, 2 LDS stages
+    {256, 128, 64, 4, 2, 3},  // 24 pipelined, 8 waves, 3 LDS stages
+    {256, 256, 64, 4, 2, 2},  // 25 pipelined, 8 waves, 2 LDS stages
+    {128, 256, 64, 2, 4, 2},  // 26 pipelined, 8 waves, 2 LDS stages
+    {512, 64, 64, 8, 1, 2},  // 27 pipelined, 8 waves, 2 LDS stages
+    {256, 64, 64, 4, 2, 3},  // 28 pipelined, 8 waves, 3 LDS stages
+    {128, 128, 64, 2, 4, 4},  // 29 pipelined, 8 waves, 4 LDS stages
 };
 constexpr int kNum = sizeof(kCfg) / sizeof(kCfg[0]);
 // configurations that also have element-load (non-16-byte) instantiations
-const bool kNoVec[kNum] = {true, false, false, true, false, false, false, true, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false};
+const bool kNoVec[kNum] = {true, false, false, true, false, false, false, true, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false};
 // configurations served by the pipelined forward kernel (lp_kernel.hpp)
-const bool kLp[kNum] = {false, false, false, false, false, false, false, false, false, true, true, true, true, true, true, true, true, true, true, true, true, true, true};
+const bool kLp[kNum] = {false, false, false, false, false, false, false, false, false, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true};
 
 }  // namespace
 
@@ -76,6 +83,13 @@ static int launch_any(int cfg, const cg_kparams *kp, const cg_group *g, int *occ
     case 20: return launch_lp<uint16_t, 64, 128, 2, 2, 2>(kp, g, occ, st);
     case 21: return launch_lp<uint16_t, 256, 128, 2, 2, 2>(kp, g, occ, st);
     case 22: return launch_lp<uint16_t, 64, 64, 2, 2, 2>(kp, g, occ, st);
+    case 23: return launch_lp<uint16_t, 256, 128, 4, 2, 2>(kp, g, occ, st);
+    case 24: return launch_lp<uint16_t, 256, 128, 4, 2, 3>(kp, g, occ, st);
+    case 25: return launch_lp<uint16_t, 256, 256, 4, 2, 2>(kp, g, occ, st);
+    case 26: return launch_lp<uint16_t, 128, 256, 2, 4, 2>(kp, g, occ, st);
+    case 27: return launch_lp<uint16_t, 512, 64, 8, 1, 2>(kp, g, occ, st);
+    case 28: return launch_lp<uint16_t, 256, 64, 4, 2, 3>(kp, g, occ, st);
+    case 29: return launch_lp<uint16_t, 128, 128, 2, 4, 4>(kp, g, occ, st);
     }
     return (int)hipErrorInvalidValue;
 }

@@ -1,5 +1,5 @@
 // igemm_f32.hip -- FP32 (v_mfma_f32_32x32x2f32) instantiations of the implicit-GEMM kernel and
-// its tile-configuration table (BM, BN, BK, waves along M, waves along N, pipelined).
+// its tile-configuration table (BM, BN, BK, waves along M, waves along N, pipelined stages).
 #include "igemm_kernel.hpp"
 #include "lp_kernel.hpp"
 #include "lp_nchw_kernel.hpp"
@@ -37,20 +37,23 @@ const cg_tilecfg kCfg[] = {
     {64, 128, 32, 2, 2, 3},  // 25 pipelined NCHW, 3 LDS stages
     {64, 64, 32, 2, 2, 4},  // 26 pipelined NCHW, 4 LDS stages
     {256, 64, 32, 4, 1, 2},  // 27 pipelined NCHW, 2 LDS stages
+    {256, 128, 32, 4, 2, 2},  // 28 pipelined, 8 waves, 2 LDS stages
+    {128, 128, 32, 2, 4, 3},  // 29 pipelined, 8 waves, 3 LDS stages
+    {256, 64, 32, 4, 2, 3},  // 30 pipelined, 8 waves, 3 LDS stages
 };
 constexpr int kNum = sizeof(kCfg) / sizeof(kCfg[0]);
 // configurations that also have element-load (non-16-byte) instantiations
 const bool kNoVec[kNum] = {true, false, false, true, false, false, true, false, false, false, false, false,
                            false, false, false, false, false, false, false, false, false, false,
-                           false, false, false, false, false, false};
+                           false, false, false, false, false, false, false, false, false};
 // configurations served by the pipelined forward kernel (lp_kernel.hpp)
 const bool kLp[kNum] = {false, false, false, false, false, false, false, false, false, false, false, false,
                         true, true, true, true, true, true, true, true, true, true,
-                        false, false, false, false, false, false};
+                        false, false, false, false, false, false, true, true, true};
 // configurations served by the pipelined forward NCHW kernel (lp_nchw_kernel.hpp)
 const bool kLpn[kNum] = {false, false, false, false, false, false, false, false, false, false, false, false,
                          false, false, false, false, false, false, false, false, false, false,
-                         true, true, true, true, true, true};
+                         true, true, true, true, true, true, false, false, false};
 
 }  // namespace
 
@@ -95,6 +98,9 @@ static int launch_any(int cfg, const cg_kparams *kp, const cg_group *g, int *occ
     case 25: return launch_lpn<64, 128, 2, 2, 3>(kp, g, occ, st);
     case 26: return launch_lpn<64, 64, 2, 2, 4>(kp, g, occ, st);
     case 27: return launch_lpn<256, 64, 4, 1, 2>(kp, g, occ, st);
+    case 28: return launch_lp<float, 256, 128, 4, 2, 2>(kp, g, occ, st);
+    case 29: return launch_lp<float, 128, 128, 2, 4, 3>(kp, g, occ, st);
+    case 30: return launch_lp<float, 256, 64, 4, 2, 3>(kp, g, occ, st);
     }
     return (int)hipErrorInvalidValue;
 }

@@ -27,7 +27,7 @@
 
 namespace cg {
 
-constexpr int LP_NT = 256;
+constexpr int LP_NT = 256;  // threads of a 4-wave workgroup (the NCHW kernel); the NHWC kernel runs 4 or 8 waves
 
 // s_waitcnt immediate: wait until at most n vector-memory operations of this
 // wave are outstanding (expcnt and lgkmcnt left at their no-wait maxima).
@@ -43,13 +43,14 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
     static_assert(sizeof(T) == 4 || sizeof(T) == 2 || sizeof(T) == 1, "FP32, BF16 or FP8 operands");
     constexpr int ES = sizeof(T);
     constexpr int RB = 128;                 // bytes per row per stage
+    constexpr int NW = WM * WN;                // 4 or 8 waves
     constexpr int WTM = BM / WM, WTN = BN / WN;
     constexpr int TR = WTM / 32, TC = WTN / 32;
-    constexpr int IA = BM / 32, IB = BN / 32;  // load instructions per wave per stage
+    constexpr int IA = BM / (8 * NW), IB = BN / (8 * NW);  // load instructions per wave per stage
     constexpr int STAGE = (BM + BN) * RB;
     constexpr int LPS = IA + IB;               // vm operations per wave per stage
-    static_assert(WM * WN == 4 && TR >= 1 && TC >= 1 && WTM % 32 == 0 && WTN % 32 == 0, "wave tile");
-    static_assert(BM % 32 == 0 && BN % 32 == 0 && NS >= 2 && NS <= 4, "tile");
+    static_assert((NW == 4 || NW == 8) && TR >= 1 && TC >= 1 && WTM % 32 == 0 && WTN % 32 == 0, "wave tile");
+    static_assert(BM % (8 * NW) == 0 && BN % (8 * NW) == 0 && NS >= 2 && NS <= 4, "tile");
     static_assert((NS - 2) * LPS < 64, "vmcnt");
 
     __shared__ __attribute__((aligned(1024))) char st0[STAGE];
@@ -365,13 +366,13 @@ __device__ inline int lp_xcd_remap(int bid, int nb) {
 }
 
 template <typename T, int BM, int BN, int WM, int WN, int NS>
-__global__ void __launch_bounds__(LP_NT) lp_kernel(const cg_kparams p) {
+__global__ void __launch_bounds__(WM * WN * 64) lp_kernel(const cg_kparams p) {
     lp_tile<T, BM, BN, WM, WN, NS>(p, lp_xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // Grouped form, with the item layout of igemm_grouped_kernel.
 template <typename T, int BM, int BN, int WM, int WN, int NS>
-__global__ void __launch_bounds__(LP_NT) lp_grouped_kernel(const cg_kparams *__restrict__ plist,
+__global__ void __launch_bounds__(WM * WN * 64) lp_grouped_kernel(const cg_kparams *__restrict__ plist,
                                                            const int *__restrict__ start, int nprob, int total) {
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         int lo = 0, hi = nprob - 1;
@@ -391,19 +392,20 @@ __global__ void __launch_bounds__(LP_NT) lp_grouped_kernel(const cg_kparams *__r
 // Forward NHWC with 16-byte channel chunks only.
 template <typename T, int BM, int BN, int WM, int WN, int NS>
 inline int launch_lp(const cg_kparams *kp, const cg_group *g, int *occ, hipStream_t st) {
+    constexpr int NT = WM * WN * 64;
     if (kp->op != CG_FWD_NHWC || !kp->vec) return (int)hipErrorInvalidValue;
     if (occ) {
         return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            occ, reinterpret_cast<const void *>(&lp_grouped_kernel<T, BM, BN, WM, WN, NS>), LP_NT, 0);
+            occ, reinterpret_cast<const void *>(&lp_grouped_kernel<T, BM, BN, WM, WN, NS>), NT, 0);
     }
     if (g) {
         if (g->total <= 0) return 0;
-        hipLaunchKernelGGL((lp_grouped_kernel<T, BM, BN, WM, WN, NS>), dim3((unsigned)g->grid), dim3(LP_NT), 0, st,
+        hipLaunchKernelGGL((lp_grouped_kernel<T, BM, BN, WM, WN, NS>), dim3((unsigned)g->grid), dim3(NT), 0, st,
                            g->plist, g->start, g->nprob, g->total);
         return (int)hipGetLastError();
     }
     const long long nb = (long long)kp->mtiles * kp->ntiles * kp->splits;
-    hipLaunchKernelGGL((lp_kernel<T, BM, BN, WM, WN, NS>), dim3((unsigned)nb), dim3(LP_NT), 0, st, *kp);
+    hipLaunchKernelGGL((lp_kernel<T, BM, BN, WM, WN, NS>), dim3((unsigned)nb), dim3(NT), 0, st, *kp);
     return (int)hipGetLastError();
 }
 
