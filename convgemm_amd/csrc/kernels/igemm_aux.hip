@@ -240,7 +240,7 @@ extern "C" int cg_igemm_occupancy(int elem, int cfg, const cg_kparams *kp0) {
 
 extern "C" int cg_igemm_sk_occupancy(int elem, int cfg, const cg_kparams *kp0) {
     cg_tilecfg t;
-    if (cg_get_tilecfg(elem, cfg, &t) || t.lp) return 0;
+    if (cg_get_tilecfg(elem, cfg, &t)) return 0;
     if (kp0->op != CG_FWD_NHWC && kp0->op != CG_FWD_NCHW && kp0->op != CG_WGRAD_NHWC && kp0->op != CG_WGRAD_NCHW)
         return 0;
     cg_kparams k = *kp0;

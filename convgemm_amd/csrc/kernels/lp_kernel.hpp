@@ -39,7 +39,8 @@ template <int N> __device__ __attribute__((always_inline)) inline void wait_vm()
 typedef __attribute__((address_space(3))) void lds_void_t;
 
 template <typename T, int BM, int BN, int WM, int WN, int NS>
-__device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &p, const int lin) {
+__device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &p, const int lin, const int sk_k0 = -1,
+                                                         const int sk_k1 = 0, float *piece = nullptr) {
     static_assert(sizeof(T) == 4 || sizeof(T) == 2 || sizeof(T) == 1, "FP32, BF16 or FP8 operands");
     constexpr int ES = sizeof(T);
     constexpr int RB = 128;                 // bytes per row per stage
@@ -108,8 +109,10 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
     // ---- reduction range (split-K) ----
     const int KT = p.KT;
     const int per_split = (KT + p.splits - 1) / p.splits;
-    const int kt_begin = min(KT, split * per_split);
-    const int kt_end = min(KT, kt_begin + per_split);
+    // stream-K (sk_k0 >= 0): `lin` is the tile and K steps [sk_k0, sk_k1) are reduced
+    const bool skm = sk_k0 >= 0;
+    const int kt_begin = skm ? sk_k0 : min(KT, split * per_split);
+    const int kt_end = skm ? sk_k1 : min(KT, kt_begin + per_split);
     const int nsteps = kt_end - kt_begin;
 
     // issue iterator: the next K step to load (tap, channel block)
@@ -350,8 +353,12 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
         for (int b = 0; b < TC; b++)
 #pragma unroll
             for (int v = 0; v < 16; v++) {
-                const int m = mt * BM + wm * WTM + a * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
-                const int n = nt * BN + wn * WTN + b * 32 + l32;
+                const int ml = wm * WTM + a * 32 + 8 * (v >> 2) + 4 * h + (v & 3), nl = wn * WTN + b * 32 + l32;
+                if (piece) {  // stream-K partial tile, [BM][BN] tile-local
+                    piece[ml * BN + nl] = acc[a][b][v];
+                    continue;
+                }
+                const int m = mt * BM + ml, n = nt * BN + nl;
                 if (m >= x.Mrows || n >= p.KN) continue;
                 if (p.splits > 1)
                     p.ws[((long long)split * p.M + m) * p.N + n] = acc[a][b][v];
@@ -368,6 +375,29 @@ __device__ inline int lp_xcd_remap(int bid, int nb) {
 template <typename T, int BM, int BN, int WM, int WN, int NS>
 __global__ void __launch_bounds__(WM * WN * 64) lp_kernel(const cg_kparams p) {
     lp_tile<T, BM, BN, WM, WN, NS>(p, lp_xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// Stream-K: the tiles x K steps of the problem are cut into gridDim.x equal
+// contiguous ranges, one per persistent workgroup.  A whole tile is written
+// back directly; a partial one (only the first and last of a range can be)
+// goes to the workgroup's piece 0 / 1, and sk_fixup_kernel sums the pieces
+// of every shared tile in workgroup order.
+template <typename T, int BM, int BN, int WM, int WN, int NS>
+__global__ void __launch_bounds__(WM * WN * 64) lp_sk_kernel(const cg_kparams p) {
+    const int G = gridDim.x;
+    const int g = lp_xcd_remap(blockIdx.x, G);
+    const long long U = (long long)p.mtiles * p.ntiles * p.KT;
+    const long long u_first = (long long)g * U / G, u_end = (long long)(g + 1) * U / G;
+    for (long long u = u_first; u < u_end;) {
+        const int t = (int)(u / p.KT);
+        const int k0 = (int)(u - (long long)t * p.KT);
+        const int k1 = (int)min((long long)p.KT, (long long)k0 + (u_end - u));
+        float *piece = (k0 == 0 && k1 == p.KT) ? nullptr
+                                                : p.ws + ((long long)g * 2 + (u == u_first ? 0 : 1)) * (BM * BN);
+        lp_tile<T, BM, BN, WM, WN, NS>(p, t, k0, k1, piece);
+        u += k1 - k0;
+        __syncthreads();  // loads, LDS reads and stores of this tile are complete
+    }
 }
 
 // Grouped form, with the item layout of igemm_grouped_kernel.
@@ -395,8 +425,13 @@ inline int launch_lp(const cg_kparams *kp, const cg_group *g, int *occ, hipStrea
     constexpr int NT = WM * WN * 64;
     if (kp->op != CG_FWD_NHWC || !kp->vec) return (int)hipErrorInvalidValue;
     if (occ) {
-        return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            occ, reinterpret_cast<const void *>(&lp_grouped_kernel<T, BM, BN, WM, WN, NS>), NT, 0);
+        const void *k = kp->sk < 0 ? reinterpret_cast<const void *>(&lp_sk_kernel<T, BM, BN, WM, WN, NS>)
+                                   : reinterpret_cast<const void *>(&lp_grouped_kernel<T, BM, BN, WM, WN, NS>);
+        return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k, NT, 0);
+    }
+    if (kp->sk > 0 && !g) {
+        hipLaunchKernelGGL((lp_sk_kernel<T, BM, BN, WM, WN, NS>), dim3((unsigned)kp->sk), dim3(NT), 0, st, *kp);
+        return (int)hipGetLastError();
     }
     if (g) {
         if (g->total <= 0) return 0;

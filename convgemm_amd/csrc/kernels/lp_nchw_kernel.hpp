@@ -19,7 +19,8 @@
 namespace cg {
 
 template <int BM, int BN, int WM, int WN, int NS>
-__device__ __attribute__((always_inline)) inline void lpn_tile(const cg_kparams &p, const int lin) {
+__device__ __attribute__((always_inline)) inline void lpn_tile(const cg_kparams &p, const int lin, const int sk_k0 = -1,
+                                                         const int sk_k1 = 0, float *piece = nullptr) {
     constexpr int RB = 128;                    // W bytes per row per stage (32 fp32)
     constexpr int KS = 32;                     // K rows per stage
     constexpr int XP = BM * 4 + 32;            // X row pitch: lanes 32-63 read 4 rows down, 32 banks over
@@ -87,8 +88,10 @@ __device__ __attribute__((always_inline)) inline void lpn_tile(const cg_kparams 
     // ---- reduction range (split-K) ----
     const int KT = p.KT;
     const int per_split = (KT + p.splits - 1) / p.splits;
-    const int kt_begin = min(KT, split * per_split);
-    const int kt_end = min(KT, kt_begin + per_split);
+    // stream-K (sk_k0 >= 0): `lin` is the tile and K steps [sk_k0, sk_k1) are reduced
+    const bool skm = sk_k0 >= 0;
+    const int kt_begin = skm ? sk_k0 : min(KT, split * per_split);
+    const int kt_end = skm ? sk_k1 : min(KT, kt_begin + per_split);
     const int nsteps = kt_end - kt_begin;
 
     int i_tap = kt_begin / p.kchunks;
@@ -280,8 +283,12 @@ __device__ __attribute__((always_inline)) inline void lpn_tile(const cg_kparams 
         for (int b = 0; b < TC; b++)
 #pragma unroll
             for (int v = 0; v < 16; v++) {
-                const int n = nt * BN + wn * WTN + a * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
-                const int m = mt * BM + wm * WTM + b * 32 + l32;
+                const int nl = wn * WTN + a * 32 + 8 * (v >> 2) + 4 * h + (v & 3), ml = wm * WTM + b * 32 + l32;
+                if (piece) {  // stream-K partial tile, [BM][BN] tile-local
+                    piece[ml * BN + nl] = acc[a][b][v];
+                    continue;
+                }
+                const int n = nt * BN + nl, m = mt * BM + ml;
                 if (m >= x.Mrows || n >= p.KN) continue;
                 if (p.splits > 1)
                     p.ws[((long long)split * p.M + m) * p.N + n] = acc[a][b][v];
@@ -293,6 +300,29 @@ __device__ __attribute__((always_inline)) inline void lpn_tile(const cg_kparams 
 template <int BM, int BN, int WM, int WN, int NS>
 __global__ void __launch_bounds__(LP_NT) lpn_kernel(const cg_kparams p) {
     lpn_tile<BM, BN, WM, WN, NS>(p, lp_xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// Stream-K: the tiles x K steps of the problem are cut into gridDim.x equal
+// contiguous ranges, one per persistent workgroup.  A whole tile is written
+// back directly; a partial one (only the first and last of a range can be)
+// goes to the workgroup's piece 0 / 1, and sk_fixup_kernel sums the pieces
+// of every shared tile in workgroup order.
+template <int BM, int BN, int WM, int WN, int NS>
+__global__ void __launch_bounds__(LP_NT) lpn_sk_kernel(const cg_kparams p) {
+    const int G = gridDim.x;
+    const int g = lp_xcd_remap(blockIdx.x, G);
+    const long long U = (long long)p.mtiles * p.ntiles * p.KT;
+    const long long u_first = (long long)g * U / G, u_end = (long long)(g + 1) * U / G;
+    for (long long u = u_first; u < u_end;) {
+        const int t = (int)(u / p.KT);
+        const int k0 = (int)(u - (long long)t * p.KT);
+        const int k1 = (int)min((long long)p.KT, (long long)k0 + (u_end - u));
+        float *piece = (k0 == 0 && k1 == p.KT) ? nullptr
+                                                : p.ws + ((long long)g * 2 + (u == u_first ? 0 : 1)) * (BM * BN);
+        lpn_tile<BM, BN, WM, WN, NS>(p, t, k0, k1, piece);
+        u += k1 - k0;
+        __syncthreads();  // loads, LDS reads and stores of this tile are complete
+    }
 }
 
 template <int BM, int BN, int WM, int WN, int NS>
@@ -318,8 +348,13 @@ template <int BM, int BN, int WM, int WN, int NS>
 inline int launch_lpn(const cg_kparams *kp, const cg_group *g, int *occ, hipStream_t st) {
     if (kp->op != CG_FWD_NCHW || !kp->vec) return (int)hipErrorInvalidValue;
     if (occ) {
-        return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            occ, reinterpret_cast<const void *>(&lpn_grouped_kernel<BM, BN, WM, WN, NS>), LP_NT, 0);
+        const void *k = kp->sk < 0 ? reinterpret_cast<const void *>(&lpn_sk_kernel<BM, BN, WM, WN, NS>)
+                                   : reinterpret_cast<const void *>(&lpn_grouped_kernel<BM, BN, WM, WN, NS>);
+        return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k, LP_NT, 0);
+    }
+    if (kp->sk > 0 && !g) {
+        hipLaunchKernelGGL((lpn_sk_kernel<BM, BN, WM, WN, NS>), dim3((unsigned)kp->sk), dim3(LP_NT), 0, st, *kp);
+        return (int)hipGetLastError();
     }
     if (g) {
         if (g->total <= 0) return 0;

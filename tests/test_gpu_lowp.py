@@ -254,3 +254,55 @@ def test_pipelined_forward_nchw_fp32(si):
             finally:
                 cg.force_config(-1, 0)
             assert rel_err(Y.cpu().numpy(), ref)[0] <= 1e-5, (cfg, splits)
+
+
+# Stream-K shapes: several tiles per workgroup range, fewer units than
+# workgroups, ragged N tiles, long K per tile.
+SK_LP_SHAPES = [
+    Shape(4, 14, 14, 64, 64, 3, 3, 1, 1, 1, 1, 1, 1),
+    Shape(2, 9, 7, 32, 20, 3, 3, 1, 1, 2, 1, 1, 1),
+    Shape(8, 28, 28, 32, 96, 1, 1, 0, 0, 1, 1, 1, 1),
+    Shape(3, 16, 16, 48, 24, 5, 5, 2, 2, 1, 1, 1, 1),
+]
+
+
+@pytest.mark.parametrize("dtype,layout", [("f32", NHWC), ("f32", NCHW), ("bf16", NHWC), ("fp8", NHWC)])
+def test_pipelined_stream_k(dtype, layout):
+    """Stream-K (split-K 0) on every pipelined configuration: pieces of tiles
+    shared between workgroups are summed by the fix-up kernel; forward with
+    bias + ReLU against the FP64 oracle."""
+    elem = {"f32": cg.F32, "bf16": cg.BF16, "fp8": cg.FP8_E4M3}[dtype]
+    cfgs = [i for i, t in enumerate(cg.tile_configs(elem)) if t[5] > 0 and cg.tile_supports(elem, i, cg.FWD, layout)]
+    assert cfgs
+    tol = {"f32": 1e-5, "bf16": 2e-5, "fp8": FP8_TOL}[dtype]
+    for si, s in enumerate(SK_LP_SHAPES):
+        if s.c % {"f32": 4, "bf16": 8, "fp8": 16}[dtype]:
+            continue
+        rng = np.random.default_rng(1300 + si)
+        X = rng.standard_normal(s.x_shape(layout)).astype(np.float32)
+        W = (rng.standard_normal(s.w_shape(layout)) * 0.1).astype(np.float32)
+        if dtype == "f32":
+            Xd, Xq, Wd, Wq, scales = torch.from_numpy(X).to(DEV), X, torch.from_numpy(W).to(DEV), W, None
+        elif dtype == "bf16":
+            (Xd, Xq), (Wd, Wq), scales = q_bf16(X), q_bf16(W), None
+        else:
+            Xd, Xq, sx = q_fp8(X)
+            Wd, Wq, sw = q_fp8(W)
+            scales = (sw, sx)
+        bias = (0.1 * rng.standard_normal(s.kn)).astype(np.float32)
+        ref = oracle.fwd(layout, s, Wq, Xq, bias=bias, relu=True)
+        geo = (s.b, s.h, s.w, s.c, s.kn, s.kh, s.kw, s.vpad, s.hpad, s.vstr, s.hstr, s.vdil, s.hdil)
+        d = cg.make_desc(cg.FWD, layout, *geo)
+        bias_d = torch.from_numpy(bias).to(DEV)
+        for cfg in cfgs:
+            cg.force_config(cfg, 0)
+            try:
+                Y = torch.full(s.y_shape(layout), float("nan"), device=DEV)
+                print(f"stream-K {dtype} layout {layout} shape {si} cfg {cfg}", flush=True)
+                cg.conv_ex(d, Wd, Xd, Y, dtype=elem, scales=scales, bias=bias_d, relu=True)
+                torch.cuda.synchronize()
+                plan = cg.last_plan()
+                assert plan[0] == cfg and plan[1] == 0, plan
+            finally:
+                cg.force_config(-1, 0)
+            assert rel_err(Y.cpu().numpy(), ref)[0] <= tol, (si, cfg)
