@@ -394,7 +394,7 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build targets one GPU; --gpus {a.gpus} is not supported "
-              "(the convolution is sharded by batch across GPUs only in design, see DESIGN.md)", file=sys.stderr)
+              "(sharding the convolution by batch across GPUs is a design, not part of this build)", file=sys.stderr)
         return 2
 
     import torch

@@ -6,11 +6,11 @@ FP32 bar: max|y - y64| / max|y64| <= 1e-5 (normwise-max; the reference's own
 element-wise check is used on its own U[0,1] data in
 test_reference_acceptance_uniform_data).
 
-Edge cases follow what the reference's tests and the survey of its code call
-for: every epilogue combination including NULL pointers, alpha/beta,
-dilation, asymmetric stride and padding, multi-block K, channel counts that
-break 16-byte vectors, kernels larger than the stride, dX accumulation,
-empty problems, and every tile configuration with and without split-K.
+Edge cases follow what the reference's tests and its code call for: every
+epilogue combination including NULL pointers, alpha/beta, dilation,
+asymmetric stride and padding, multi-block K, channel counts that break
+16-byte vectors, kernels larger than the stride, dX accumulation, empty
+problems, and every tile configuration with and without split-K.
 """
 import numpy as np
 import pytest
