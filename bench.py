@@ -231,9 +231,11 @@ def batch_sweep(torch, cg, stream, n=256, reps=10, seed=5):
     Cin/Cout 32..1024, k {1,3,5}, s {1,2}): one conv_ex call per problem vs
     one batch (grouped launches captured in a hipGraph), plans from the
     learned cost model shipped in convgemm_amd/data (trained on autotuner
-    timings).  Returns aggregate TFLOP/s of each way of launching."""
+    timings).  Returns aggregate TFLOP/s of each way of launching, and the
+    learned and analytic plans' regret against exhaustive autotuning of the
+    same problems."""
     from convgemm_amd import tuning
-    model = os.path.join(ROOT, "convgemm_amd", "data", "mi355x_bf16_fwd_nhwc.mlp")
+    model = os.path.join(ROOT, "convgemm_amd", "data", "mi355x_costmodel.mlp")
     cg.costmodel_load(model)
     cg.set_planner(cg.PLAN_MODEL)
     try:
@@ -277,6 +279,20 @@ def batch_sweep(torch, cg, stream, n=256, reps=10, seed=5):
                         "kernels_per_run": kern}
             b.close()
         res["speedup_graph_vs_sequential"] = round(res["sequential_calls"]["ms"] / res["batch_graph"]["ms"], 2)
+        # exhaustive autotuning of the same problems (untimed) judges the learned plans:
+        # regret t(chosen) / t(best) from the recorded timings, and the batch again on
+        # the measured-best plans
+        t0 = time.time()
+        for d, w, x, y in ops:
+            cg.autotune(d, w, x, y, dtype=cg.BF16, reps=3, stream=stream)
+        res["autotune_seconds"] = round(time.time() - t0, 1)
+        res["regret_learned_vs_exhaustive"] = tuning.regret(probs, cg.BF16, cg.PLAN_MODEL)
+        res["regret_analytic_vs_exhaustive"] = tuning.regret(probs, cg.BF16, cg.PLAN_ANALYTIC)
+        cg.set_planner(cg.PLAN_TUNED)
+        b = cg.Batch([cg.problem(d, w, x, y) for d, w, x, y in ops], cg.BF16, cg.BATCH_GRAPH)
+        ms = timed(lambda: b.run(stream))
+        res["batch_graph_exhaustive_plans"] = {"ms": round(ms, 4), "tflops": round(flops / (ms * 1e-3) / 1e12, 2)}
+        b.close()
         return res
     finally:
         cg.set_planner(cg.PLAN_TUNED)

@@ -43,8 +43,10 @@ EXPORTED_SYMBOLS = (
     "convgemm_version", "convgemm_quantize", "convgemm_packed_weights_size", "convgemm_pack_weights",
     "convgemm_conv_packed",
     # convgemm_tune.h
-    "convgemm_set_planner", "convgemm_autotune", "convgemm_tune_count", "convgemm_tune_save", "convgemm_tune_load",
-    "convgemm_tune_clear", "convgemm_plan", "convgemm_costmodel_train", "convgemm_costmodel_save",
+    "convgemm_set_planner", "convgemm_get_planner", "convgemm_autotune", "convgemm_tune_count", "convgemm_tune_save",
+    "convgemm_tune_load",
+    "convgemm_tune_clear", "convgemm_plan", "convgemm_costmodel_train", "convgemm_costmodel_set_progress",
+    "convgemm_costmodel_save",
     "convgemm_costmodel_load", "convgemm_costmodel_predict",
     # convgemm_batch.h
     "convgemm_batch_create", "convgemm_batch_run", "convgemm_batch_info", "convgemm_batch_destroy",
@@ -55,6 +57,8 @@ EXPORTED_SYMBOLS = (
 )
 
 PLAN_ANALYTIC, PLAN_TUNED, PLAN_LEARNED, PLAN_MODEL = 0, 1, 2, 3
+# CONVGEMM_MAX_CANDIDATES of convgemm_tune.h: candidate plans of one problem at most
+MAX_CANDIDATES = 1024
 
 
 class ConvGemmError(RuntimeError):
@@ -138,6 +142,8 @@ def lib() -> ctypes.CDLL:
     L.convgemm_quantize.restype = ci
     L.convgemm_set_planner.argtypes = [ci]
     L.convgemm_set_planner.restype = None
+    L.convgemm_get_planner.argtypes = []
+    L.convgemm_get_planner.restype = ci
     L.convgemm_autotune.argtypes = [ctypes.POINTER(Desc), ci, vp, vp, vp, vp, ci, ctypes.POINTER(ci),
                                     ctypes.POINTER(ci), ctypes.POINTER(cf), vp]
     L.convgemm_autotune.restype = ci
@@ -153,6 +159,8 @@ def lib() -> ctypes.CDLL:
     L.convgemm_plan.restype = ci
     L.convgemm_costmodel_train.argtypes = [ci, cf, ctypes.c_uint, ctypes.POINTER(cf), vp]
     L.convgemm_costmodel_train.restype = ci
+    L.convgemm_costmodel_set_progress.argtypes = [_TRAIN_PROGRESS, ci, vp]
+    L.convgemm_costmodel_set_progress.restype = None
     L.convgemm_costmodel_save.argtypes = [ctypes.c_char_p]
     L.convgemm_costmodel_save.restype = ci
     L.convgemm_costmodel_load.argtypes = [ctypes.c_char_p]
@@ -192,6 +200,8 @@ def lib() -> ctypes.CDLL:
               "convgemm_weights_nchw_to_nhwc"):
         getattr(L, f).argtypes = [vp, vp, ci, ci, ci, ci, ci, vp]
         getattr(L, f).restype = ci
+    L.cg_split_factors.argtypes = [ctypes.POINTER(ci), ci]
+    L.cg_split_factors.restype = ci
     L.cg_num_tilecfgs.argtypes = [ci]
     L.cg_num_tilecfgs.restype = ci
     L.cg_get_tilecfg.argtypes = [ci, ci, ctypes.POINTER(ci * 6)]
@@ -348,8 +358,14 @@ def _scales_ptr(scales):
 
 
 def set_planner(mode: int) -> None:
-    """PLAN_ANALYTIC, PLAN_TUNED (default), PLAN_LEARNED or PLAN_MODEL."""
+    """PLAN_ANALYTIC, PLAN_TUNED, PLAN_LEARNED or PLAN_MODEL.  The library starts with
+    PLAN_LEARNED when it loaded a cost model at start-up (the shipped one, or the file
+    CONVGEMM_COSTMODEL names), PLAN_TUNED otherwise."""
     lib().convgemm_set_planner(int(mode))
+
+
+def get_planner() -> int:
+    return lib().convgemm_get_planner()
 
 
 def autotune(desc: Desc, in_, x, out, dtype=F32, scales=None, reps=3, stream=None):
@@ -394,16 +410,27 @@ def tune_clear() -> None:
 def tune_get(desc: Desc, dtype=F32):
     """Recorded timings of a tuned problem: list of (cfg, splits, microseconds)."""
     L = lib()
-    c, s, u = (ctypes.c_int * 256)(), (ctypes.c_int * 256)(), (ctypes.c_float * 256)()
-    n = L.cg_tune_get(ctypes.byref(desc), int(dtype), c, s, u, 256)
+    m = MAX_CANDIDATES
+    c, s, u = (ctypes.c_int * m)(), (ctypes.c_int * m)(), (ctypes.c_float * m)()
+    n = L.cg_tune_get(ctypes.byref(desc), int(dtype), c, s, u, m)
     return [(c[i], s[i], u[i]) for i in range(n)]
 
 
-def costmodel_train(epochs=1500, lr=3e-3, seed=0, stream=None) -> float:
-    """Train the cost model on every timing in the tuning database (device kernels); returns the final loss."""
+_TRAIN_PROGRESS = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_float, ctypes.c_void_p)
+
+
+def costmodel_train(epochs=1500, lr=3e-3, seed=0, stream=None, progress=None, every=100) -> float:
+    """Train the cost model on every timing in the tuning database (device kernels); returns the final loss.
+    progress(epoch, loss), if given, is called every `every` epochs."""
     L = lib()
     loss = ctypes.c_float()
-    st = L.convgemm_costmodel_train(int(epochs), float(lr), int(seed), ctypes.byref(loss), _stream_handle(stream))
+    cb = _TRAIN_PROGRESS(lambda ep, l, _ctx: progress(ep, l)) if progress else _TRAIN_PROGRESS()
+    L.convgemm_costmodel_set_progress(cb, int(every) if progress else 0, None)
+    try:
+        st = L.convgemm_costmodel_train(int(epochs), float(lr), int(seed), ctypes.byref(loss),
+                                        _stream_handle(stream))
+    finally:
+        L.convgemm_costmodel_set_progress(_TRAIN_PROGRESS(), 0, None)
     if st != 0:
         raise ConvGemmError(st, L.convgemm_last_error_message().decode())
     return loss.value
@@ -422,8 +449,9 @@ def costmodel_load(path: str) -> None:
 def costmodel_predict(desc: Desc, dtype=F32):
     """Predicted microseconds of every candidate: list of (cfg, splits, us)."""
     L = lib()
-    c, s, u = (ctypes.c_int * 256)(), (ctypes.c_int * 256)(), (ctypes.c_float * 256)()
-    n = ctypes.c_int(256)
+    m = MAX_CANDIDATES
+    c, s, u = (ctypes.c_int * m)(), (ctypes.c_int * m)(), (ctypes.c_float * m)()
+    n = ctypes.c_int(m)
     st = L.convgemm_costmodel_predict(ctypes.byref(desc), int(dtype), c, s, u, ctypes.byref(n))
     if st != 0:
         raise ConvGemmError(st, L.convgemm_last_error_message().decode())
@@ -606,6 +634,13 @@ def tile_configs(dtype=F32):
         if L.cg_get_tilecfg(int(dtype), c, ctypes.byref(v)) == 0:
             out.append(tuple(v))
     return out
+
+
+def split_factors():
+    """Split-K factors the tuner and the cost model try for every configuration."""
+    buf = (ctypes.c_int * 64)()
+    n = lib().cg_split_factors(buf, 64)
+    return [buf[i] for i in range(min(n, 64))]
 
 
 def tile_supports(dtype, cfg, op, layout, vec=True) -> bool:

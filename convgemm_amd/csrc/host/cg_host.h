@@ -37,7 +37,9 @@ convgemm_status_t cg_set_error(convgemm_status_t st, const char *msg);
 convgemm_status_t cg_prepare(const convgemm_desc_t *d, convgemm_dtype_t dtype, const void *in, const void *x,
                              const void *wsw, cg_kparams *kp, int *elem, int *chunk);
 /* Candidate plans of a prepared problem: every supported configuration with
- * split-K factors that leave >= 2 K tiles per split.  Returns the count. */
+ * split-K factors that leave >= 2 K tiles per split, plus one stream-K plan
+ * (splits 0) per configuration.  At most (split factors + 1) x configurations,
+ * which CONVGEMM_MAX_CANDIDATES bounds.  Returns the count. */
 int cg_candidates(int elem, const cg_kparams *kp, int *cfgs, int *splits, int max);
 
 /* Plan (cfg, splits) as a call would: forced plan, planner hook, analytic model. */

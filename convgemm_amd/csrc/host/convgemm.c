@@ -453,6 +453,11 @@ convgemm_status_t cg_prepare(const convgemm_desc_t *d, convgemm_dtype_t dtype, c
     return CONVGEMM_SUCCESS;
 }
 
+/* Set while a call plans for a stream that is being captured: the planner
+ * hook may then not run device work (the cost model's inference). */
+static _Thread_local int t_in_capture;
+int cg_planning_in_capture(void) { return t_in_capture; }
+
 /* Plan (cfg, splits) for a prepared problem: forced plan, planner hook, analytic model. */
 static int choose_plan(int elem, const cg_kparams *kp, int *cfg, int *splits) {
     *cfg = -1;
@@ -559,7 +564,14 @@ static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, 
 
     /* plan on the first chunk's shape */
     int cfg, splits;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        cs = hipStreamCaptureStatusNone;
+    }
+    t_in_capture = cs != hipStreamCaptureStatusNone;
     choose_plan(elem, &kp, &cfg, &splits);
+    t_in_capture = 0;
     cg_tilecfg t;
     if (cfg < 0 || cg_get_tilecfg(elem, cfg, &t))
         return set_error(CONVGEMM_ERR_UNSUPPORTED, "no kernel configuration for this problem");

@@ -23,7 +23,7 @@
 #include "../cg_internal.h"
 #include "cg_host.h"
 
-#define CM_F 28
+#define CM_F 30
 #define CM_H 64
 #define CM_NP (CM_F * CM_H + CM_H + CM_H * CM_H + CM_H + CM_H + 1)
 #define OFF_W1 0
@@ -97,6 +97,8 @@ int cg_features(int elem, const cg_kparams *kp0, int cfg, int splits, float *f) 
     f[24] = lg((double)((wgs + 256LL * per_cu - 1) / (256LL * per_cu)));
     f[25] = lg(2.0 * (double)kp.M * kp.N * (double)kp.KT * t.BK);
     f[26] = t.lp ? 1.0f : 0.0f;
+    f[28] = lg(t.lp ? (double)t.WM * t.WN : 4.0); /* waves per workgroup (4 or 8) */
+    f[29] = (float)t.lp;                           /* LDS stages of the pipelined kernels */
     if (splits == 0) { /* stream-K: the persistent workgroups, no split partials */
         const int g = cg_sk_groups(elem, cfg, &kp);
         f[22] = lg((double)g);
@@ -259,8 +261,25 @@ static double gauss(uint64_t *s) {
     return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
 }
 
+static convgemm_train_progress_t g_progress;
+static int g_progress_every;
+static void *g_progress_ctx;
+
+void convgemm_costmodel_set_progress(convgemm_train_progress_t fn, int every, void *ctx) {
+    pthread_mutex_lock(&m_lock);
+    g_progress = fn;
+    g_progress_every = every;
+    g_progress_ctx = ctx;
+    pthread_mutex_unlock(&m_lock);
+}
+
 convgemm_status_t convgemm_costmodel_train(int epochs, float lr, unsigned seed, float *loss,
                                            convgemm_stream_t stream) {
+    pthread_mutex_lock(&m_lock);
+    const convgemm_train_progress_t progress = g_progress;
+    const int every = g_progress_every;
+    void *const pctx = g_progress_ctx;
+    pthread_mutex_unlock(&m_lock);
     tset ts = {0};
     cg_tune_foreach(collect, &ts);
     if (ts.n < 8) {
@@ -320,6 +339,8 @@ convgemm_status_t convgemm_costmodel_train(int epochs, float lr, unsigned seed, 
     for (int ep = 1; ep <= epochs && !e; ep++) {
         e = forward(&d, st);
         e = e ? e : backward(&d, st);
+        /* the loss partials of this epoch's parameters are in d.part after backward */
+        if (!e && progress && every > 0 && ep % every == 0) progress(ep, read_loss(&d, st), pctx);
         e = e ? e : cg_mlp_adam(d.P, d.G, d.M, d.V, CM_NP, lr, 0.9f, 0.999f, 1e-8f, ep, st);
     }
     float final_loss = NAN;
@@ -375,7 +396,7 @@ static int predict(int elem, const cg_kparams *kp, const int *cfgs, const int *s
     return 0;
 }
 
-int cg_costmodel_choose(int elem, const cg_kparams *kp, int *cfg, int *splits) {
+int cg_costmodel_choose(int elem, const cg_kparams *kp, int *cfg, int *splits, int allow_device) {
     int key[16];
     cg_key_of(elem, kp, key);
     uint64_t h = 1469598103934665603ull;
@@ -390,10 +411,10 @@ int cg_costmodel_choose(int elem, const cg_kparams *kp, int *cfg, int *splits) {
         return 0;
     }
     pthread_mutex_unlock(&m_lock);
-    if (!ready) return -1;
-    int cfgs[256], spl[256];
-    float us[256];
-    int n = cg_candidates(elem, kp, cfgs, spl, 256);
+    if (!ready || !allow_device) return -1;
+    int cfgs[CONVGEMM_MAX_CANDIDATES], spl[CONVGEMM_MAX_CANDIDATES];
+    float us[CONVGEMM_MAX_CANDIDATES];
+    int n = cg_candidates(elem, kp, cfgs, spl, CONVGEMM_MAX_CANDIDATES);
     if (n <= 0 || predict(elem, kp, cfgs, spl, n, us)) return -1;
     int bi = 0;
     for (int i = 1; i < n; i++)
@@ -416,9 +437,9 @@ convgemm_status_t convgemm_costmodel_predict(const convgemm_desc_t *desc, convge
     convgemm_status_t st = cg_prepare(desc, dtype, (void *)256, (void *)256, NULL, &kp, &elem, &chunk);
     if (st != CONVGEMM_SUCCESS) return st;
     if (!g_model.ready) return cg_set_error(CONVGEMM_ERR_INVALID_VALUE, "cost model not trained or loaded");
-    int c[256], s[256];
-    float u[256];
-    int n = cg_candidates(elem, &kp, c, s, 256);
+    int c[CONVGEMM_MAX_CANDIDATES], s[CONVGEMM_MAX_CANDIDATES];
+    float u[CONVGEMM_MAX_CANDIDATES];
+    int n = cg_candidates(elem, &kp, c, s, CONVGEMM_MAX_CANDIDATES);
     if (n > *n_out) n = *n_out;
     if (predict(elem, &kp, c, s, n, u)) return cg_set_error(CONVGEMM_ERR_HIP, "cost model inference failed");
     for (int i = 0; i < n; i++) {
@@ -432,7 +453,7 @@ convgemm_status_t convgemm_costmodel_predict(const convgemm_desc_t *desc, convge
 
 /* -------------------------------------------------------------- files */
 
-static const char MAGIC[8] = {'C', 'G', 'M', 'L', 'P', '0', '0', '3'};
+static const char MAGIC[8] = {'C', 'G', 'M', 'L', 'P', '0', '0', '4'};
 
 convgemm_status_t convgemm_costmodel_save(const char *path) {
     pthread_mutex_lock(&m_lock);

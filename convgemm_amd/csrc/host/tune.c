@@ -7,6 +7,8 @@
  * every candidate on the device, keeps every timing (the cost model's
  * training data) and serves the fastest as the plan for the tuned shape.
  */
+#define _GNU_SOURCE /* dladdr */
+#include <dlfcn.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdint.h>
@@ -22,7 +24,7 @@
 #include "cg_host.h"
 
 #define KEYLEN 16
-#define MAXCAND 256
+#define MAXCAND CONVGEMM_MAX_CANDIDATES
 
 typedef struct {
     int key[KEYLEN]; /* elem, op, vec, B, H, W, C, KN, KH, KW, vp, hp, vs, hs, vd, hd */
@@ -143,8 +145,17 @@ int cg_tune_foreach(void (*fn)(void *ctx, const int *key, int n, const int *cfg,
 
 /* ------------------------------------------------------------ candidates */
 
+/* Split-K factors tried for every configuration. */
+static const int S[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256};
+
+/* Test hook: the split-K factors (count returned, up to max written). */
+int cg_split_factors(int *out, int max) {
+    int n = (int)(sizeof(S) / sizeof(S[0]));
+    for (int i = 0; i < n && i < max; i++) out[i] = S[i];
+    return n;
+}
+
 int cg_candidates(int elem, const cg_kparams *base, int *cfgs, int *splits, int max) {
-    static const int S[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256};
     int n = 0, ncfg = cg_num_tilecfgs(elem);
     for (int c = 0; c < ncfg; c++) {
         cg_tilecfg t;
@@ -177,7 +188,8 @@ int cg_candidates(int elem, const cg_kparams *base, int *cfgs, int *splits, int 
 
 /* --------------------------------------------------------------- planner */
 
-int cg_costmodel_choose(int elem, const cg_kparams *kp, int *cfg, int *splits);
+int cg_costmodel_choose(int elem, const cg_kparams *kp, int *cfg, int *splits, int allow_device);
+int cg_planning_in_capture(void);
 
 static void plan_hook(void *ctx, int elem, const cg_kparams *kp, int *cfg, int *splits) {
     (void)ctx;
@@ -197,7 +209,9 @@ static void plan_hook(void *ctx, int elem, const cg_kparams *kp, int *cfg, int *
         }
         pthread_mutex_unlock(&t_lock);
     }
-    if (t_mode == CONVGEMM_PLAN_LEARNED || t_mode == CONVGEMM_PLAN_MODEL) cg_costmodel_choose(elem, kp, cfg, splits);
+    /* under stream capture only plans the model already made (no device inference) */
+    if (t_mode == CONVGEMM_PLAN_LEARNED || t_mode == CONVGEMM_PLAN_MODEL)
+        cg_costmodel_choose(elem, kp, cfg, splits, !cg_planning_in_capture());
 }
 
 void convgemm_set_planner(convgemm_planner_t mode) {
@@ -205,8 +219,32 @@ void convgemm_set_planner(convgemm_planner_t mode) {
     cg_set_plan_hook(plan_hook, NULL);
 }
 
+convgemm_planner_t convgemm_get_planner(void) { return t_mode; }
+
+/* The cost model loaded at start-up: CONVGEMM_COSTMODEL names a file ("" or
+ * "none": no model), else data/mi355x_costmodel.mlp next to this library.
+ * With a model the default planner is CONVGEMM_PLAN_LEARNED. */
+static void load_default_model(void) {
+    const char *env = getenv("CONVGEMM_COSTMODEL");
+    char path[4096];
+    if (env) {
+        if (!*env || !strcmp(env, "none")) return;
+        snprintf(path, sizeof(path), "%s", env);
+    } else {
+        Dl_info info;
+        if (!dladdr((void *)&load_default_model, &info) || !info.dli_fname) return;
+        snprintf(path, sizeof(path), "%s", info.dli_fname);
+        char *slash = strrchr(path, '/');
+        size_t dir = slash ? (size_t)(slash - path + 1) : 0;
+        snprintf(path + dir, sizeof(path) - dir, "data/mi355x_costmodel.mlp");
+    }
+    if (convgemm_costmodel_load(path) == CONVGEMM_SUCCESS) t_mode = CONVGEMM_PLAN_LEARNED;
+    cg_set_error(CONVGEMM_SUCCESS, ""); /* a missing file is not the caller's error */
+}
+
 __attribute__((constructor)) static void tune_init(void) {
     cg_set_plan_hook(plan_hook, NULL);
+    load_default_model();
     const char *db = getenv("CONVGEMM_TUNING_DB");
     if (db && *db) convgemm_tune_load(db);
 }

@@ -19,9 +19,15 @@ extern "C" {
 
 /* Planner used by every convolution call:
  *   CONVGEMM_PLAN_ANALYTIC: the analytic model only;
- *   CONVGEMM_PLAN_TUNED:    tuning database, else analytic (default);
+ *   CONVGEMM_PLAN_TUNED:    tuning database, else analytic;
  *   CONVGEMM_PLAN_LEARNED:  tuning database, else the cost model, else analytic;
- *   CONVGEMM_PLAN_MODEL:    the cost model, else analytic (ignores the database). */
+ *   CONVGEMM_PLAN_MODEL:    the cost model, else analytic (ignores the database).
+ * At start-up the library loads a cost model from the file CONVGEMM_COSTMODEL
+ * names ("" or "none": none), else from data/mi355x_costmodel.mlp next to the
+ * library; the default planner is CONVGEMM_PLAN_LEARNED when one loaded,
+ * CONVGEMM_PLAN_TUNED otherwise.  The model runs on the device, once per new
+ * problem shape (its plans are cached); a call on a stream that is being
+ * captured only reuses cached plans and otherwise plans without the model. */
 typedef enum {
     CONVGEMM_PLAN_ANALYTIC = 0,
     CONVGEMM_PLAN_TUNED = 1,
@@ -30,6 +36,7 @@ typedef enum {
 } convgemm_planner_t;
 
 void convgemm_set_planner(convgemm_planner_t mode);
+convgemm_planner_t convgemm_get_planner(void);
 
 /* Time every supported (tile configuration, split-K) pair on this problem with
  * the caller's buffers (the output is overwritten; backward data accumulates
@@ -55,8 +62,17 @@ convgemm_status_t convgemm_plan(const convgemm_desc_t *desc, convgemm_dtype_t dt
  * (forward, backward, Adam).  Returns the final training loss in *loss. */
 convgemm_status_t convgemm_costmodel_train(int epochs, float lr, unsigned seed, float *loss,
                                            convgemm_stream_t stream);
+/* Optional progress report for convgemm_costmodel_train: fn(epoch, standardized
+ * training loss, ctx) every `every` epochs (fn NULL or every <= 0: none).  Each
+ * report synchronizes the training stream once. */
+typedef void (*convgemm_train_progress_t)(int epoch, float loss, void *ctx);
+void convgemm_costmodel_set_progress(convgemm_train_progress_t fn, int every, void *ctx);
 convgemm_status_t convgemm_costmodel_save(const char *path);
 convgemm_status_t convgemm_costmodel_load(const char *path);
+/* Upper bound on the candidate plans (configuration, split) of one problem:
+ * arrays of this length hold every candidate of any problem. */
+#define CONVGEMM_MAX_CANDIDATES 1024
+
 /* Predicted time (microseconds) of every candidate of a problem; out[i] is
  * for (cfgs[i], splits[i]).  n_out: capacity in, candidates written out. */
 convgemm_status_t convgemm_costmodel_predict(const convgemm_desc_t *desc, convgemm_dtype_t dtype, int *cfgs,

@@ -8,7 +8,7 @@ row-major.  Pinned in tests/test_oracle.py by central finite differences.
 """
 import numpy as np
 
-F, H = 28, 64
+F, H = 30, 64
 NP = F * H + H + H * H + H + H + 1
 
 

@@ -10,6 +10,7 @@ import ctypes
 import os
 import re
 import subprocess
+import sys
 import tempfile
 
 import pytest
@@ -26,6 +27,7 @@ def header_functions():
             continue
         txt = open(os.path.join(ROOT, "include", fn)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        txt = re.sub(r"typedef\s+\w+\s*\(\s*\*\s*\w+\s*\)\s*\([^;]*\);", "", txt)  # callback types
         for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt):
             name = m.group(1)
             if name in ("if", "sizeof", "defined"):
@@ -177,3 +179,56 @@ def test_fastdiv_exact():
         assert r.returncode == 0, r.stdout
         out = subprocess.run([exe], stdout=subprocess.PIPE, text=True)
         assert out.returncode == 0 and out.stdout.startswith("OK"), out.stdout
+
+
+def test_candidate_capacity_covers_every_tile_table():
+    """Every configuration with every split-K factor, plus one stream-K plan per
+    configuration, fits CONVGEMM_MAX_CANDIDATES: the autotuner and the cost
+    model rank the whole candidate list of a problem, never a prefix of it."""
+    hdr = open(os.path.join(ROOT, "include", "convgemm_tune.h")).read()
+    m = re.search(r"#define\s+CONVGEMM_MAX_CANDIDATES\s+(\d+)", hdr)
+    assert m and int(m.group(1)) == cg.MAX_CANDIDATES
+    per_cfg = len(cg.split_factors()) + 1
+    assert per_cfg > 2
+    for dt in (cg.F32, cg.BF16, cg.FP8_E4M3):
+        n = len(cg.tile_configs(dt))
+        assert n > 0 and n * per_cfg <= cg.MAX_CANDIDATES, (dt, n, per_cfg)
+
+
+def test_mix_spec_parsing():
+    """--mix entries of the tuning module: dtype:op:layout with an optional training count."""
+    from convgemm_amd.tuning import parse_mix
+    m = parse_mix("f32:fwd:nchw,bf16:wgrad:nhwc:2000", 150)
+    assert m == [("f32:fwd:nchw", cg.F32, cg.FWD, cg.NCHW, 150), ("bf16:wgrad:nhwc", cg.BF16, cg.WGRAD, cg.NHWC, 2000)]
+    with pytest.raises(ValueError):
+        parse_mix("bf16:fwd", 10)
+
+
+SHIPPED_MODEL = os.path.join(ROOT, "convgemm_amd", "data", "mi355x_costmodel.mlp")
+
+
+@pytest.mark.parametrize("env,expect", [(None, "PLAN_LEARNED"), ("none", "PLAN_TUNED"), ("", "PLAN_TUNED"),
+                                        ("/nonexistent/model.mlp", "PLAN_TUNED"), (SHIPPED_MODEL, "PLAN_LEARNED")])
+def test_startup_planner_follows_cost_model(env, expect):
+    """The library loads the shipped cost model (or CONVGEMM_COSTMODEL's file) when it
+    is loaded, and then plans with PLAN_LEARNED; without a model, PLAN_TUNED.  A missing
+    model leaves no error behind.  Each case runs in a fresh process: loading happens once."""
+    assert os.path.exists(SHIPPED_MODEL)
+    e = dict(os.environ)
+    e.pop("CONVGEMM_COSTMODEL", None)
+    if env is not None:
+        e["CONVGEMM_COSTMODEL"] = env
+    code = ("import convgemm_amd as cg; L = cg.lib(); "
+            "print(cg.get_planner(), cg." + expect + ", L.convgemm_get_last_error())")
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=e, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout
+    got, want, err = r.stdout.split()[-3:]
+    assert got == want and err == "0", r.stdout
+
+
+def test_shipped_cost_model_file_loads():
+    """The shipped model has this build's feature count and file format."""
+    cg.costmodel_load(SHIPPED_MODEL)
+    with open(SHIPPED_MODEL, "rb") as f:
+        assert f.read(8) == b"CGMLP004"

@@ -26,13 +26,20 @@ from convgemm_amd import tuning  # noqa: E402
 from tests.gpu_util import DEV  # noqa: E402
 
 
+SHIPPED_MODEL = os.path.join(os.path.dirname(cg.__file__), "data", "mi355x_costmodel.mlp")
+
+
 @pytest.fixture(autouse=True)
 def _clean_tuning_state():
+    """Each test starts from an empty database; afterwards the shipped model and the
+    start-up planner are back, for the tests that follow."""
+    start = cg.get_planner()
     cg.tune_clear()
     cg.set_planner(cg.PLAN_TUNED)
     yield
     cg.tune_clear()
-    cg.set_planner(cg.PLAN_TUNED)
+    cg.costmodel_load(SHIPPED_MODEL)
+    cg.set_planner(start)
 
 
 @pytest.mark.parametrize("n", [1, 37, 300])
@@ -69,7 +76,10 @@ def test_train_save_load_and_plan(tmp_path):
     n_rows = sum(len(cg.tune_get(tuning.desc_of(p), cg.BF16)) for p in probs)
     assert n_rows >= 24
     short = cg.costmodel_train(epochs=5, lr=3e-3, seed=1)
-    loss = cg.costmodel_train(epochs=600, lr=3e-3, seed=1)
+    seen = []
+    loss = cg.costmodel_train(epochs=600, lr=3e-3, seed=1, every=200, progress=lambda ep, l: seen.append((ep, l)))
+    assert [ep for ep, _ in seen] == [200, 400, 600] and all(np.isfinite(l) for _, l in seen), seen
+    assert seen[-1][1] < seen[0][1] < short, (seen, short)
     assert np.isfinite(loss) and loss < short and loss < 0.5, (short, loss)  # targets are standardized: 1.0 = mean
     d = tuning.desc_of(probs[0])
     pred = cg.costmodel_predict(d, cg.BF16)
@@ -119,3 +129,19 @@ def test_learned_planner_regret_vs_analytic():
     assert r_model["n"] == 48 and r_an["n"] == 48
     assert r_fit["geomean"] < 1.15, r_fit
     assert r_model["geomean"] < r_an["geomean"], (r_model, r_an)
+
+
+@pytest.mark.parametrize("mix", ["f32:fwd:nchw", "f32:dgrad:nhwc", "bf16:wgrad:nhwc", "fp8:fwd:nhwc"])
+def test_shipped_model_beats_analytic(mix):
+    """The model shipped with the library (the start-up default), on problems it was
+    not trained on: every plan it picks is among the timed candidates, and its
+    geomean regret against exhaustive tuning is below the analytic planner's."""
+    (_, dt, op, lay, _), = tuning.parse_mix(mix, 0)
+    probs = tuning.random_problems(24, seed=4242, dtype=dt, op=op, layout=lay, **SMALL)
+    tuning.tune_all(probs, reps=3, log_every=0)
+    cg.costmodel_load(SHIPPED_MODEL)
+    r_m = tuning.regret(probs, None, cg.PLAN_MODEL)
+    r_a = tuning.regret(probs, None, cg.PLAN_ANALYTIC)
+    print(mix, "regret model", r_m, "analytic", r_a, flush=True)
+    assert r_m["n"] == 24 and r_m["outside"] == 0
+    assert r_m["geomean"] < r_a["geomean"], (r_m, r_a)
