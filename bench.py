@@ -9,10 +9,11 @@ with the fused bias + batch-norm + ReLU epilogue: 10 convolutions, 57.6 GFLOP
 Inputs are synthetic (X ~ N(0,1), He-scaled weights, epilogue vectors as in
 the workload description) and resident in HBM before timing starts.
 
-Plans: by default the tuning database shipped in convgemm_amd/data (the
-autotuner's timings of these ten problems on an MI355X, the plans the
-committed profiles were measured with) is loaded read-only; problems it
-lacks, or every problem with --fresh-tune, are autotuned at setup (untimed).
+Plans: by default the tuning databases shipped in convgemm_amd/data (the
+autotuner's timings of these ten problems, and of their backward, on an
+MI355X: the plans the committed profiles were measured with) are loaded
+read-only; problems they lack, or every problem with --fresh-tune, are
+autotuned at setup (untimed).
 
 Prints one JSON line:  value = TFLOP/s over the timed steps, the roofline
 of the dominant kernel measured with HIP events on the launch stream, the
@@ -33,6 +34,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 SHIPPED_TUNE_DB = os.path.join(ROOT, "convgemm_amd", "data", "mi355x_fp32_sweep_b32.tunedb")
+# backward (weight gradient, backward data; FP32 and BF16) of the same layers at batch 32
+SHIPPED_BWD_DB = os.path.join(ROOT, "convgemm_amd", "data", "mi355x_backward_b32.tunedb")
 sys.path.insert(0, ROOT)
 
 METRIC = "conv TFLOP/s (% of roofline), BF16/FP8 vs FP32 plain at stated FP64 error"
@@ -164,7 +167,8 @@ def precision_sweep(torch, cg, stream, tune=True, reps=10, batch=64):
 def backward_sweep(torch, cg, stream, reps=10, batch=32):
     """Backward of the layer sweep at batch 32, NHWC: weight gradient
     dW = im2row(X)^T dY and backward data dX += row2im(W^T dY), FP32 and BF16
-    operands (FP32 accumulation), autotuned; TFLOP/s of the convolution
+    operands (FP32 accumulation), plans from the loaded tuning database or
+    autotuned at setup; TFLOP/s of the convolution
     kernels and the error against the FP64 oracle at sampled outputs (BF16
     against the unquantized FP32 data)."""
     from oracle import Shape, sampled
@@ -197,7 +201,8 @@ def backward_sweep(torch, cg, stream, reps=10, batch=32):
             dx_desc = cg.make_desc(cg.DGRAD, 0, batch, h, w, c, kn, kh, kw, pad, pad, st, st)
             out = {}
             for op, desc, args in (("wgrad", dw_desc, (dYo, Xo, dW)), ("dgrad", dx_desc, (Wo, dYo, dX))):
-                cg.autotune(desc, *args, dtype=dt, reps=3, stream=stream)
+                if not cg.tune_get(desc, dt):  # not in a loaded database: autotune (untimed)
+                    cg.autotune(desc, *args, dtype=dt, reps=3, stream=stream)
                 for _ in range(2):
                     cg.conv_ex(desc, *args, beta=1.0 if op == "dgrad" else 0.0, dtype=dt, stream=stream)
                 torch.cuda.synchronize()
@@ -209,7 +214,7 @@ def backward_sweep(torch, cg, stream, reps=10, batch=32):
                 torch.cuda.synchronize()
                 ms = e0.elapsed_time(e1) / reps
                 out[op] = {"ms": round(ms, 4), "tflops": round(flops / (ms * 1e-3) / 1e12, 2),
-                           "plan": list(cg.last_plan()[:2])}
+                           "plan": list(cg.last_plan())}  # cfg, splits, kernels per call
             # errors of one clean evaluation of each
             cg.conv_ex(dw_desc, dYo, Xo, dW, dtype=dt, stream=stream)
             dX.zero_()
@@ -405,6 +410,8 @@ def main(argv=None):
                     help="only time hipBLASLt (torch.matmul) on the layers' GEMM shapes, for calibration")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB per step (calibrates HBM byte counters)")
+    ap.add_argument("--backward-only", action="store_true",
+                    help="only run the backward sweep and print it (for profiling its kernels)")
     ap.add_argument("--only", default="", help="comma-separated layer names (e.g. L2_3x3_64to64_56_NHWC) "
                     "to run alone, for profiling; the JSON value then covers only those")
     a = ap.parse_args(argv)
@@ -429,7 +436,15 @@ def main(argv=None):
         plans = f"tuning database {os.path.basename(a.tune_db)}, then autotuned"
     elif not a.tune_db and not a.fresh_tune and not a.no_tune and os.path.exists(SHIPPED_TUNE_DB):
         cg.tune_load(SHIPPED_TUNE_DB)
+        if os.path.exists(SHIPPED_BWD_DB):
+            cg.tune_load(SHIPPED_BWD_DB)
         plans = "shipped tuning database (MI355X autotuner timings)"
+    if a.backward_only:
+        res = {"backward_sweep": backward_sweep(torch, cg, stream), "plans": plans}
+        if a.tune_db:
+            cg.tune_save(a.tune_db)
+        print(json.dumps(res), flush=True)
+        return 0
     convs = []
     for layout in (0, 1):
         for li, (name, spec) in enumerate(LAYERS):

@@ -398,6 +398,8 @@ static int vec_ok(int elem, const convgemm_desc_t *d, const void *a, const void 
     if (((uintptr_t)a % 16) || ((uintptr_t)b % 16)) return 0;
     if (d->op == CONVGEMM_FWD) return d->c % vw == 0;
     if (d->op == CONVGEMM_DGRAD) return d->kn % vw == 0;
+    /* weight gradient, NHWC: 16-byte groups of channels of X and of kn of dY */
+    if (d->layout == CONVGEMM_NHWC) return d->c % vw == 0 && d->kn % vw == 0;
     return 1;
 }
 

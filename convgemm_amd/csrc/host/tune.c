@@ -369,9 +369,15 @@ convgemm_status_t convgemm_tune_save(const char *path) {
 convgemm_status_t convgemm_tune_load(const char *path) {
     FILE *f = fopen(path, "r");
     if (!f) return cg_set_error(CONVGEMM_ERR_INVALID_VALUE, "cannot open tuning database");
-    char line[1 << 14];
+    /* a line holds at most CONVGEMM_MAX_CANDIDATES triples of at most 48 characters */
+    const int line_max = 1 << 16;
+    char *line = malloc(line_max);
+    if (!line) {
+        fclose(f);
+        return cg_set_error(CONVGEMM_ERR_ALLOC, "tuning database: out of host memory");
+    }
     int loaded = 0;
-    while (fgets(line, sizeof(line), f)) {
+    while (fgets(line, line_max, f)) {
         if (line[0] == '#' || line[0] == '\n') continue;
         int k[KEYLEN], n, pos = 0, used;
         char *p = line;
@@ -395,6 +401,7 @@ convgemm_status_t convgemm_tune_load(const char *path) {
         loaded++;
     }
     fclose(f);
+    free(line);
     (void)loaded;
     return cg_set_error(CONVGEMM_SUCCESS, "");
 }

@@ -57,6 +57,66 @@ template <int KIND, int R, int KQ> constexpr bool loader_ok() {
     return (R * KQ) % NT == 0;
 }
 
+// Weight gradient, NHWC, 16-byte operands ("RV" loader): both operands are
+// contiguous along their row dimension (channels of X, kn of dY) and the K
+// dimension is pixels.  A thread loads one 16-byte group of VW rows at J
+// consecutive pixels (J = (R/VW)*BK/NT vector loads) and writes it transposed:
+// VW rows, each a K-contiguous run of J elements.  Consecutive lanes read
+// consecutive 16-byte groups of one pixel (coalesced).  In LDS the 16-byte
+// chunk index of a row is XOR-swizzled by (row >> 3), which spreads the
+// transposed stores over the banks; 8 consecutive rows share a swizzle, so the
+// MFMA operand reads stay conflict-free.
+template <int R, int BK, int VW> constexpr bool rv_ok() {
+    if (R % VW != 0 || R / VW == 0 || NT % (R / VW) != 0 || (R / VW) * BK % NT != 0) return false;
+    const int J = (R / VW) * BK / NT;
+    return J >= 1 && (J <= VW ? VW % J == 0 : J % VW == 0);
+}
+
+template <int KQ, int VW>
+__device__ inline int rv_k(int row, int k) {  // LDS element offset of K index k within `row`
+    return ((k / VW) ^ ((row >> 3) & (KQ - 1))) * VW + (k % VW);
+}
+
+template <int NB>
+__device__ inline void lds_put(void *dst, const void *src) {
+    if constexpr (NB == 16) {
+        i32x4 w;
+        __builtin_memcpy(&w, src, 16);
+        *(i32x4 *)dst = w;
+    } else if constexpr (NB == 8) {
+        i32x2 w;
+        __builtin_memcpy(&w, src, 8);
+        *(i32x2 *)dst = w;
+    } else if constexpr (NB == 4) {
+        int w;
+        __builtin_memcpy(&w, src, 4);
+        *(int *)dst = w;
+    } else {
+        static_assert(NB == 2, "piece size");
+        unsigned short w;
+        __builtin_memcpy(&w, src, 2);
+        *(unsigned short *)dst = w;
+    }
+}
+
+// Element r of each of the J chunks in `src` forms row (row0 + r)'s K run
+// [k0, k0 + J), written in pieces of at most one 16-byte chunk.
+template <typename T, int J, int VW, int PITCH, int KQ, typename V>
+__device__ __attribute__((always_inline)) inline void rv_store(T *dst, int row0, int k0, const V (&src)[J]) {
+    constexpr int PIECE = J < VW ? J : VW;
+#pragma unroll
+    for (int r = 0; r < VW; r++) {
+        const int row = row0 + r;
+#pragma unroll
+        for (int h = 0; h < J; h += PIECE) {
+            T v[PIECE];
+#pragma unroll
+            for (int i = 0; i < PIECE; i++) v[i] = src[h + i][r];
+            lds_put<PIECE * (int)sizeof(T)>(dst + row * PITCH + rv_k<KQ, VW>(row, k0 + h), v);
+        }
+    }
+}
+
 // Thread -> (row, chunk) of a loader.
 template <int KIND, int R, int KQ>
 __device__ inline int ld_row(int tid, int i) {
@@ -94,6 +154,10 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
     constexpr int NRA = (AK == LK_RS) ? 1 : PA;  // distinct A rows per thread
     constexpr int NRB = (BKIND == LK_RS) ? 1 : PB;
     static_assert(loader_ok<AK, BM, KQ>() && loader_ok<BKIND, BN, KQ>(), "loader geometry");
+    // RV loader (weight gradient, NHWC, 16-byte operands); its per-thread chunk
+    // counts are PA and PB, the same as the element loader's
+    constexpr bool RV = OP == CG_WGRAD_NHWC && VEC && rv_ok<BM, BK, VW>() && rv_ok<BN, BK, VW>();
+    static_assert(!RV || (KQ & (KQ - 1)) == 0, "RV swizzle needs a power-of-two chunk count");
     static_assert(BK % G == 0, "BK must be a multiple of the MFMA group");
     constexpr int WTM = BM / WM, WTN = BN / WN;
     constexpr int ER = SW ? WTN : WTM, EC = SW ? WTM : WTN;
@@ -158,7 +222,7 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
     int a_b[NRA], a_h[NRA], a_w[NRA];  // FWD: b, x*vs-vp, y*hs-hp; DGRAD: b, i', j'; WGRAD: -, c, -
 #pragma unroll
     for (int i = 0; i < NRA; i++) {
-        const int row = ld_row<AK, BM, KQ>(tid, i);
+        const int row = RV ? (tid % (BM / VW)) * VW : ld_row<AK, BM, KQ>(tid, i);
         a_b[i] = 0;
         a_h[i] = FAR;
         a_w[i] = FAR;
@@ -190,7 +254,7 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
     int b_row[NRB];  // FWD/WGRAD: kn index (or -1); DGRAD: c index (or -1)
 #pragma unroll
     for (int i = 0; i < NRB; i++) {
-        const int r = nt * BN + ld_row<BKIND, BN, KQ>(tid, i);
+        const int r = nt * BN + (RV ? (tid % (BN / VW)) * VW : ld_row<BKIND, BN, KQ>(tid, i));
         b_row[i] = r < (DG ? p.C : p.KN) ? r : -1;
     }
 
@@ -308,7 +372,30 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
             // WGRAD: K = output pixels p0 .. p0+BK
             const int p0 = kt * BK;
             const int P = (int)p.P;
-            if constexpr (OP == CG_WGRAD_NHWC) {
+            if constexpr (RV) {
+                const int c = a_h[0];  // first channel of this thread's 16-byte group (or -1)
+                const int pa = p0 + (tid / (BM / VW)) * PA;
+                int b = 0, xo = 0, yo = 0;
+                if (pa < P) {
+                    b = fdiv(pa, x.howo);
+                    const int pix = pa - b * HoWo;
+                    xo = fdiv(pix, x.wo);
+                    yo = pix - xo * p.Wo;
+                }
+#pragma unroll
+                for (int i = 0; i < PA; i++) {
+                    const int ih = xo * p.vs - p.vp + wg_kx * p.vd;
+                    const int iw = yo * p.hs - p.hp + wg_ky * p.hd;
+                    const bool ok = c >= 0 && pa + i < P && (unsigned)ih < (unsigned)p.H && (unsigned)iw < (unsigned)p.W;
+                    ra[i] = ld_vec<T>(rA, ok ? ((((unsigned)b * p.H + ih) * p.W + iw) * p.C + c) * ES : OOB);
+                    if (++yo == p.Wo) { yo = 0; if (++xo == p.Ho) { xo = 0; ++b; } }
+                }
+                const int n = b_row[0];
+                const int pb = p0 + (tid / (BN / VW)) * PB;
+#pragma unroll
+                for (int i = 0; i < PB; i++)
+                    rb[i] = ld_vec<T>(rB, (n >= 0 && pb + i < P) ? ((unsigned)(pb + i) * p.KN + n) * ES : OOB);
+            } else if constexpr (OP == CG_WGRAD_NHWC) {
                 const int c = a_h[0];
 #pragma unroll
                 for (int i = 0; i < PA; i++) {
@@ -380,6 +467,11 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
     auto store_tile = [&](int buf, const V (&ra)[PA], const V (&rb)[PB]) __attribute__((always_inline)) {
         T *dA = sA + buf * BM * PITCH;
         T *dB = sB + buf * BN * PITCH;
+        if constexpr (RV) {
+            rv_store<T, PA, VW, PITCH, KQ>(dA, (tid % (BM / VW)) * VW, (tid / (BM / VW)) * PA, ra);
+            rv_store<T, PB, VW, PITCH, KQ>(dB, (tid % (BN / VW)) * VW, (tid / (BN / VW)) * PB, rb);
+            return;
+        }
 #pragma unroll
         for (int i = 0; i < PA; i++)
             *(V *)(dA + ld_row<AK, BM, KQ>(tid, i) * PITCH + ld_kq<AK, BM, KQ>(tid, i) * VW) = ra[i];
@@ -400,6 +492,9 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
 #pragma unroll
             for (int v = 0; v < 16; v++) acc[a][b][v] = 0.f;
 
+    // LDS element offset of K index k (a multiple of 4 elements) within row r
+    auto kof = [](int r, int k) __attribute__((always_inline)) { return RV ? rv_k<KQ, VW>(r, k) : k; };
+
     auto compute_tile = [&](int buf) __attribute__((always_inline)) {
         const T *Lr = SW ? sB + buf * BN * PITCH : sA + buf * BM * PITCH;
         const T *Lc = SW ? sA + buf * BM * PITCH : sB + buf * BN * PITCH;
@@ -414,15 +509,15 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
                 f32x4 ar[TR][2], ac[TC][2];
 #pragma unroll
                 for (int a = 0; a < TR; a++) {
-                    const float *s = Lr + (roff + a * 32 + l32) * PITCH + g + hoff;
-                    ar[a][0] = *(const f32x4 *)s;
-                    ar[a][1] = *(const f32x4 *)(s + 4);
+                    const int r = roff + a * 32 + l32;
+                    ar[a][0] = *(const f32x4 *)(Lr + r * PITCH + kof(r, g + hoff));
+                    ar[a][1] = *(const f32x4 *)(Lr + r * PITCH + kof(r, g + hoff + 4));
                 }
 #pragma unroll
                 for (int b = 0; b < TC; b++) {
-                    const float *s = Lc + (coff + b * 32 + l32) * PITCH + g + hoff;
-                    ac[b][0] = *(const f32x4 *)s;
-                    ac[b][1] = *(const f32x4 *)(s + 4);
+                    const int r = coff + b * 32 + l32;
+                    ac[b][0] = *(const f32x4 *)(Lc + r * PITCH + kof(r, g + hoff));
+                    ac[b][1] = *(const f32x4 *)(Lc + r * PITCH + kof(r, g + hoff + 4));
                 }
 #pragma unroll
                 for (int s = 0; s < 8; s++)
@@ -435,9 +530,15 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
             } else if constexpr (std::is_same<T, uint16_t>::value) {
                 u16x8 ar[TR], ac[TC];
 #pragma unroll
-                for (int a = 0; a < TR; a++) ar[a] = *(const u16x8 *)(Lr + (roff + a * 32 + l32) * PITCH + g + hoff);
+                for (int a = 0; a < TR; a++) {
+                    const int r = roff + a * 32 + l32;
+                    ar[a] = *(const u16x8 *)(Lr + r * PITCH + kof(r, g + hoff));
+                }
 #pragma unroll
-                for (int b = 0; b < TC; b++) ac[b] = *(const u16x8 *)(Lc + (coff + b * 32 + l32) * PITCH + g + hoff);
+                for (int b = 0; b < TC; b++) {
+                    const int r = coff + b * 32 + l32;
+                    ac[b] = *(const u16x8 *)(Lc + r * PITCH + kof(r, g + hoff));
+                }
 #pragma unroll
                 for (int a = 0; a < TR; a++)
 #pragma unroll
@@ -610,7 +711,7 @@ template <typename T> constexpr bool op_enabled(int op) {
 // Does the VEC flag change any loader of this operation?
 template <typename T> constexpr bool vec_relevant(int op) {
     return op == CG_FWD_NHWC || (op == CG_FWD_NCHW && fwd_b_kmajor<T>()) || op == CG_DGRAD_NHWC ||
-           op == CG_DGRAD_NCHW;
+           op == CG_DGRAD_NCHW || op == CG_WGRAD_NHWC;
 }
 
 // g == nullptr: one problem (kp), one workgroup per item.  g != nullptr: the

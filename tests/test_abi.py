@@ -232,3 +232,25 @@ def test_shipped_cost_model_file_loads():
     cg.costmodel_load(SHIPPED_MODEL)
     with open(SHIPPED_MODEL, "rb") as f:
         assert f.read(8) == b"CGMLP004"
+
+
+def test_tuning_database_holds_full_candidate_lists(tmp_path):
+    """A database line with CONVGEMM_MAX_CANDIDATES timings (over 16 KB of text, with
+    the 1e30 placeholder the save writes for failed candidates) loads whole."""
+    d = cg.make_desc(cg.FWD, cg.NHWC, 2, 8, 8, 16, 16, 3, 3, 1, 1, 1, 1)
+    key = "0 0 1 2 8 8 16 16 3 3 1 1 1 1 1 1"  # F32, forward NHWC, 16-byte loads, then the shape
+    n = cg.MAX_CANDIDATES
+    us = [1e30 if i % 4 == 1 else 1000.0 + i / 7 for i in range(n)]
+    line = key + f" {n} " + " ".join(f"{i % 31} {1 + i % 200} {u:.3f}" for i, u in enumerate(us))
+    assert len(line) > 1 << 14
+    path = tmp_path / "long.tunedb"
+    path.write_text("# header\n" + line + "\n")
+    cg.tune_clear()
+    try:
+        cg.tune_load(str(path))
+        got = cg.tune_get(d, cg.F32)
+    finally:
+        cg.tune_clear()
+    assert len(got) == n
+    assert [(c, s) for c, s, _ in got] == [(i % 31, 1 + i % 200) for i in range(n)]
+    assert all(abs(g - u) <= 1e-3 * u for (_, _, g), u in zip(got, us))

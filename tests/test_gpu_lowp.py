@@ -170,6 +170,45 @@ def test_every_lowp_configuration(dtype):
                 assert rel_err(y, ref)[0] <= (2e-5 if dtype == "bf16" else FP8_TOL), (layout, cfg, splits)
 
 
+@pytest.mark.parametrize("splits", [1, 3, 0])
+def test_every_bf16_configuration_backward(splits):
+    """Weight gradient and backward data through every BF16 configuration that
+    offers them: 16-byte channel groups (C, kn multiples of 8; the weight
+    gradient's transposing loader in NHWC) and element loads (C = 12), with
+    split-K and stream-K (splits 0, weight gradient)."""
+    ncfg = len(cg.tile_configs(cg.BF16))
+    cases = [(Shape(2, 11, 10, 64, 80, 3, 3, 1, 1, 1, 1, 1, 1), True),
+             (Shape(2, 9, 8, 12, 24, 3, 3, 1, 1, 2, 1, 1, 1), False),
+             (Shape(3, 7, 7, 40, 16, 1, 1, 0, 0, 1, 1, 1, 1), True)]
+    for ci, (s, vec) in enumerate(cases):
+        rng = np.random.default_rng(700 + ci)
+        geo = (s.b, s.h, s.w, s.c, s.kn, s.kh, s.kw, s.vpad, s.hpad, s.vstr, s.hstr, s.vdil, s.hdil)
+        for layout in (NHWC, NCHW):
+            X = rng.standard_normal(s.x_shape(layout)).astype(np.float32)
+            W = (rng.standard_normal(s.w_shape(layout)) * 0.1).astype(np.float32)
+            dY = rng.standard_normal(s.y_shape(layout)).astype(np.float32)
+            (Xd, Xq), (Wd, Wq), (Yd, Yq) = q_bf16(X), q_bf16(W), q_bf16(dY)
+            ref_w = oracle.wgrad(layout, s, Yq, Xq)
+            ref_x = oracle.dgrad(layout, s, Wq, Yq)
+            for cfg in range(ncfg):
+                for op in (cg.WGRAD, cg.DGRAD):
+                    if not cg.tile_supports(cg.BF16, cfg, op, layout, vec):
+                        continue
+                    print(f"bf16 backward cfg={cfg} splits={splits} op={op} layout={layout} {s}", flush=True)
+                    cg.force_config(cfg, splits)
+                    try:
+                        if op == cg.WGRAD:
+                            got = _run(cg.make_desc(op, layout, *geo), Yd, Xd, s.w_shape(layout), cg.BF16)
+                            ref = ref_w
+                        else:
+                            got = _run(cg.make_desc(op, layout, *geo), Wd, Yd, s.x_shape(layout), cg.BF16)
+                            ref = ref_x
+                        assert cg.last_plan()[0] == cfg
+                    finally:
+                        cg.force_config(-1, 0)
+                    assert rel_err(got, ref)[0] <= 2e-5, (cfg, splits, op, layout, ci)
+
+
 # Shapes for the pipelined forward kernel (128-byte K steps, 2-4 LDS stages):
 # a single K step, every tail length, ragged channel blocks, padding, stride,
 # dilation, M and N tails.
