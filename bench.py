@@ -41,6 +41,33 @@ sys.path.insert(0, ROOT)
 METRIC = "conv TFLOP/s (% of roofline), BF16/FP8 vs FP32 plain at stated FP64 error"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+PEAK_TFLOPS = {"fp32": PEAK_FP32_TFLOPS, "bf16": 2500.0, "fp8": 5000.0}  # MI355X dense MFMA
+ESIZE = {"fp32": 4, "bf16": 2, "fp8": 1}
+
+
+def roofline_of(flops, nbytes, dname, ms):
+    """Attainable TFLOP/s = min(dense MFMA peak of the operand type, AI x 8 TB/s) for
+    `nbytes` compulsory HBM bytes, and the fractions of it and of the peak reached in `ms`."""
+    tf = flops / (ms * 1e-3) / 1e12
+    hbm = flops / nbytes * PEAK_HBM_GBS * 1e9 / 1e12
+    att = min(PEAK_TFLOPS[dname], hbm)
+    return {"bound": "mfma" if att == PEAK_TFLOPS[dname] else "hbm", "attainable_tflops": round(att, 1),
+            "frac_roofline": round(tf / att, 3), "frac_peak": round(tf / PEAK_TFLOPS[dname], 4)}
+
+
+def op_bytes(spec, op, dname):
+    """Compulsory HBM bytes of one operation with operands of type dname and an FP32 output:
+    forward |X| + |W| (+ |Y|), weight gradient |X| + |dY| (+ |dW|), backward data |dY| + |W|
+    (+ 2|dX|: read and written, it accumulates)."""
+    b, h, w, c, kn, kh, kw, pad, st = spec
+    ho, wo = out_hw(h, pad, kh, st), out_hw(w, pad, kw, st)
+    nx, nw, ny = b * h * w * c, c * kh * kw * kn, b * ho * wo * kn
+    e = ESIZE[dname]
+    if op == "fwd":
+        return e * (nx + nw) + 4.0 * ny
+    if op == "wgrad":
+        return e * (nx + ny) + 4.0 * nw
+    return e * (ny + nw) + 8.0 * nx
 
 # name: (b, h, w, c, kn, kh, kw, pad, stride)
 LAYERS = [
@@ -156,6 +183,7 @@ def precision_sweep(torch, cg, stream, tune=True, reps=10, batch=64):
             err = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
             rel2 = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
             row[name_dt] = {"ms": round(ms, 4), "tflops": round(flops / (ms * 1e-3) / 1e12, 2),
+                            "roofline": roofline_of(flops, op_bytes(spec, "fwd", name_dt), name_dt, ms),
                             "fp64_max_rel_err": err, "fp64_l2_rel_err": rel2,
                             "quantize_ms": round(conv_ms, 4), "plan": list(cg.last_plan()[:2])}
         for k in ("bf16", "fp8"):
@@ -214,6 +242,7 @@ def backward_sweep(torch, cg, stream, reps=10, batch=32):
                 torch.cuda.synchronize()
                 ms = e0.elapsed_time(e1) / reps
                 out[op] = {"ms": round(ms, 4), "tflops": round(flops / (ms * 1e-3) / 1e12, 2),
+                           "roofline": roofline_of(flops, op_bytes((batch,) + spec[1:], op, dname), dname, ms),
                            "plan": list(cg.last_plan())}  # cfg, splits, kernels per call
             # errors of one clean evaluation of each
             cg.conv_ex(dw_desc, dYo, Xo, dW, dtype=dt, stream=stream)

@@ -46,6 +46,8 @@ typedef struct {
     int KT;          /* K tiles of one output tile (DGRAD: upper bound, all taps) */
     int splits;      /* split-K factor (1: no partial buffer) */
     int ncls_h, ncls_w; /* DGRAD residue classes (vs, hs); 1, 1 otherwise */
+    unsigned long long cls_order; /* DGRAD with <= 16 classes: the i-th class in launch order is
+                                     (cls_order >> 4i) & 15, most taps first; 0 otherwise (identity) */
     int vec;         /* reduction channels are a multiple of the 16-byte chunk (see cg_vec_ok) */
     long long P;     /* B*Ho*Wo */
     const void *A;   /* FWD/WGRAD: X;  DGRAD: dY */

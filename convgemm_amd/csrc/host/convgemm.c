@@ -273,11 +273,43 @@ static int opkind(const convgemm_desc_t *d) {
 
 static int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
+/* Taps of kernel rows (or columns) that reach input positions of residue r
+ * modulo stride s: (r + pad - k * dil) divisible by s. */
+static int class_taps(int r, int pad, int k, int dil, int s) {
+    int n = 0;
+    for (int i = 0; i < k; i++)
+        if ((r + pad - i * dil) % s == 0) n++;
+    return n;
+}
+
+/* Backward-data launch order of the residue classes: most taps (the longest
+ * workgroups) first, ties in class order; 0 (identity) above 16 classes. */
+static unsigned long long dgrad_class_order(const cg_kparams *kp) {
+    const int n = kp->vs * kp->hs;
+    if (n <= 1 || n > 16) return 0;
+    int cls[16], w[16];
+    for (int c = 0; c < n; c++) {
+        cls[c] = c;
+        w[c] = class_taps(c / kp->hs, kp->vp, kp->KH, kp->vd, kp->vs) *
+               class_taps(c % kp->hs, kp->hp, kp->KW, kp->hd, kp->hs);
+    }
+    for (int i = 1; i < n; i++) /* stable insertion sort, descending taps */
+        for (int j = i; j > 0 && w[cls[j]] > w[cls[j - 1]]; j--) {
+            int t = cls[j];
+            cls[j] = cls[j - 1];
+            cls[j - 1] = t;
+        }
+    unsigned long long ord = 0;
+    for (int i = 0; i < n; i++) ord |= (unsigned long long)cls[i] << (4 * i);
+    return ord;
+}
+
 /* Fill the tile-space fields of kp for configuration t. */
 void cg_fill_tiles(cg_kparams *kp, const cg_tilecfg *t, int splits) {
     int op = kp->op;
     kp->ncls_h = 1;
     kp->ncls_w = 1;
+    kp->cls_order = 0;
     if (op == CG_FWD_NHWC || op == CG_FWD_NCHW) {
         kp->M = (int)kp->P;
         kp->N = kp->KN;
@@ -286,6 +318,7 @@ void cg_fill_tiles(cg_kparams *kp, const cg_tilecfg *t, int splits) {
     } else if (op == CG_DGRAD_NHWC || op == CG_DGRAD_NCHW) {
         kp->ncls_h = kp->vs;
         kp->ncls_w = kp->hs;
+        kp->cls_order = dgrad_class_order(kp);
         int hc = cdiv(kp->H, kp->vs), wc = cdiv(kp->W, kp->hs); /* class (0,0) is the largest */
         kp->M = kp->B * hc * wc;
         kp->N = kp->C;

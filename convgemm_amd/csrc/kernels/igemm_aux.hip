@@ -9,26 +9,55 @@
 
 namespace cg {
 
-// ---- split-K reduction: one thread per (class, m, n) ----
+// ---- split-K reduction + epilogue ----
+// A 256-thread block reduces 256 / G consecutive outputs of one class, with G
+// split groups per output: group g sums the splits g, g + G, g + 2G, ... and
+// group 0 adds the other groups' sums in order (a fixed summation order, the
+// same in the single and the grouped launch).  Deep splits over small outputs
+// (the weight gradient: fewer than 512 blocks of 256 outputs) get G = 4: four
+// times the blocks and a quarter of the serial chain of one thread per output.
+// Large outputs keep G = 1 (there the extra blocks' set-up cost more).
+__host__ __device__ inline int reduce_groups(const cg_kparams &p) {
+    return p.splits >= 8 && (long long)p.M * p.N < 512LL * 256 ? 4 : 1;
+}
+
 template <int OP>
-__device__ __attribute__((always_inline)) inline void reduce_elem(const cg_kparams &p, int cls, long long idx) {
+__device__ __attribute__((always_inline)) inline void reduce_block(const cg_kparams &p, int cls, long long blk) {
+    __shared__ float part[3][64];
+    const int G = reduce_groups(p);  // uniform over the block
+    const int per = 256 / G;
+    const int g = (int)threadIdx.x / per, e = (int)threadIdx.x - g * per;
     const Ctx x = make_ctx(p, cls);
     const long long total = (long long)x.Mrows * p.N;
-    if (idx >= total) return;
-    const int m = (int)(idx / p.N), n = (int)(idx - (long long)m * p.N);
+    const long long idx = blk * per + e;
+    const bool live = idx < total;
+    const int m = live ? (int)(idx / p.N) : 0, n = live ? (int)(idx - (long long)m * p.N) : 0;
     const int ncls = p.ncls_h * p.ncls_w;
     float s = 0.f;
-    for (int z = 0; z < p.splits; z++) s += p.ws[(((long long)z * ncls + cls) * p.M + m) * p.N + n];
-    store_out<OP>(p, x, m, n, s);
+    if (live)
+        for (int z = g; z < p.splits; z += G) s += p.ws[(((long long)z * ncls + cls) * p.M + m) * p.N + n];
+    if (G == 1) {
+        if (live) store_out<OP>(p, x, m, n, s);
+        return;
+    }
+    if (g > 0) part[g - 1][e] = s;
+    __syncthreads();
+    if (g == 0 && live) {
+        s += part[0][e];
+        s += part[1][e];
+        s += part[2][e];
+        store_out<OP>(p, x, m, n, s);
+    }
+    __syncthreads();  // `part` is reused by the next item of a grouped launch
 }
 
 template <int OP>
 __global__ void __launch_bounds__(256) splitk_reduce_kernel(const cg_kparams p) {
-    reduce_elem<OP>(p, blockIdx.y, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+    reduce_block<OP>(p, blockIdx.y, blockIdx.x);
 }
 
-// Grouped: item = one 256-element block of one class of one problem; the
-// blocks of a class cover M*N (the largest class), as in the single launch.
+// Grouped: item = one block of one class of one problem; the blocks of a
+// class cover M*N (the largest class), as in the single launch.
 template <int OP>
 __global__ void __launch_bounds__(256) splitk_reduce_grouped_kernel(const cg_kparams *__restrict__ plist,
                                                                     const int *__restrict__ start, int nprob,
@@ -42,9 +71,10 @@ __global__ void __launch_bounds__(256) splitk_reduce_grouped_kernel(const cg_kpa
         }
         const cg_kparams &p = plist[lo];
         const int local = item - start[lo];
-        const int per_cls = (int)(((long long)p.M * p.N + 255) / 256);
+        const int per = 256 / reduce_groups(p);
+        const int per_cls = (int)(((long long)p.M * p.N + per - 1) / per);
         const int cls = local / per_cls, blk = local - cls * per_cls;
-        reduce_elem<OP>(p, cls, (long long)blk * 256 + threadIdx.x);
+        reduce_block<OP>(p, cls, blk);
     }
 }
 
@@ -267,7 +297,8 @@ extern "C" int cg_launch_sk_fixup(const cg_kparams *kp, int BM, int BN, void *st
 }
 
 extern "C" long long cg_reduce_items(const cg_kparams *kp) {
-    return (long long)kp->ncls_h * kp->ncls_w * (((long long)kp->M * kp->N + 255) / 256);
+    const int per = 256 / reduce_groups(*kp);
+    return (long long)kp->ncls_h * kp->ncls_w * (((long long)kp->M * kp->N + per - 1) / per);
 }
 
 extern "C" int cg_launch_splitk_reduce_grouped(int op, const cg_group *g, void *stream) {
@@ -330,7 +361,8 @@ extern "C" int cg_launch_igemm(int elem, int cfg, const cg_kparams *kp, void *st
 extern "C" int cg_launch_splitk_reduce(const cg_kparams *kp, void *stream) {
     hipStream_t st = (hipStream_t)stream;
     const long long total = (long long)kp->M * kp->N;  // rows of the largest class bound every class
-    dim3 grid((unsigned)((total + 255) / 256), kp->ncls_h * kp->ncls_w);
+    const int per = 256 / reduce_groups(*kp);  // outputs per block
+    dim3 grid((unsigned)((total + per - 1) / per), kp->ncls_h * kp->ncls_w);
     switch (kp->op) {
     case CG_FWD_NHWC: launch_reduce<CG_FWD_NHWC>(kp, grid, st); break;
     case CG_FWD_NCHW: launch_reduce<CG_FWD_NCHW>(kp, grid, st); break;

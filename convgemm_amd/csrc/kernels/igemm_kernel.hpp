@@ -170,14 +170,22 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
-    // ---- item -> (tile, split, class).  The N tiles of one M panel are
+    // ---- item -> (class, tile, split).  The N tiles of one M panel are
     // adjacent: the gathered operand (the large one) is then fetched from HBM
     // once and the small weight panel stays L2-resident (with M fastest the
-    // gathered operand was measured re-read once per N tile).
+    // gathered operand was measured re-read once per N tile).  Backward data:
+    // the residue classes of one tile are adjacent items, so every XCD's run
+    // of items (xcd_remap) holds all classes in equal shares -- with the class
+    // outermost, each XCD ran one class and the XCDs with the most taps per
+    // class set the time -- and within a tile the class with the most taps
+    // comes first (cls_order).
     const int tiles = p.mtiles * p.ntiles;
     const bool skm = sk_k0 >= 0;
-    const int tile = skm ? lin : lin % tiles, rest = skm ? 0 : lin / tiles;
-    const int split = rest % p.splits, cls = rest / p.splits;
+    const int ncls_all = p.ncls_h * p.ncls_w;
+    const int cslot = skm ? 0 : lin % ncls_all, rest0 = skm ? 0 : lin / ncls_all;
+    const int tile = skm ? lin : rest0 % tiles, rest = skm ? 0 : rest0 / tiles;
+    const int split = rest % p.splits;
+    const int cls = p.cls_order ? (int)((p.cls_order >> (4 * cslot)) & 15) : cslot;
     const int nt = tile % p.ntiles, mt = tile / p.ntiles;
 
     const Ctx x = make_ctx(p, cls);

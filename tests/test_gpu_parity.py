@@ -120,6 +120,47 @@ def test_every_tile_configuration(cfg, splits):
         cg.force_config(-1, 0)
 
 
+@pytest.mark.parametrize("splits", [8, 16, 40])
+def test_deep_split_k_reduction(splits):
+    """Split-K with 8 or more slices reduces each output in four interleaved groups
+    (combined in a fixed order): weight gradient in both layouts, forward with the
+    epilogue, strided backward data, against the oracle; and a one-problem batch
+    (the grouped reduction) bit-identical to the single call."""
+    s = Shape(4, 28, 28, 64, 40, 3, 3, 1, 1, 1, 1, 1, 1)
+    s2 = Shape(2, 13, 12, 40, 64, 3, 3, 1, 1, 2, 2, 1, 1)
+    cg.force_config(3, splits)  # 64x64, BK 32
+    try:
+        for layout in (NHWC, NCHW):
+            rng = np.random.default_rng(900 + splits + layout)
+            X, W, dY, epi = _case(rng, s, layout)
+            print(f"deep split-K {splits} layout={layout} {s}", flush=True)
+            dW = run_wgrad(layout, s, dY, X)
+            assert cg.last_plan()[:2] == (3, splits)
+            assert rel_err(dW, oracle.wgrad(layout, s, dY, X))[0] <= TOL
+        rng = np.random.default_rng(950 + splits)
+        X, W, dY, epi = _case(rng, s, NHWC)
+        assert rel_err(run_fwd(NHWC, s, W, X, epi), oracle.fwd(NHWC, s, W, X, **epi))[0] <= TOL
+        X2, W2, dY2, _ = _case(rng, s2, NHWC)
+        dX0 = rng.standard_normal(s2.x_shape(NHWC)).astype(np.float32)
+        got = run_dgrad(NHWC, s2, W2, dY2, dX0)
+        assert rel_err(got, oracle.dgrad(NHWC, s2, W2, dY2, dX0=dX0))[0] <= TOL
+        # the grouped reduction of a batch sums in the same order as the single launch
+        d = cg.make_desc(cg.WGRAD, NHWC, s.b, s.h, s.w, s.c, s.kn, s.kh, s.kw, s.vpad, s.hpad, s.vstr, s.hstr)
+        dYd, Xd = dev(dY), dev(X)
+        single = torch.empty(s.w_shape(NHWC), device=DEV)
+        cg.conv_ex(d, dYd, Xd, single)
+        batched = torch.full(s.w_shape(NHWC), float("nan"), device=DEV)
+        b = cg.Batch([cg.problem(d, dYd, Xd, batched)], cg.F32, cg.BATCH_GRAPH)
+        try:
+            b.run()
+            torch.cuda.synchronize()
+        finally:
+            b.close()
+        assert torch.equal(single, batched)
+    finally:
+        cg.force_config(-1, 0)
+
+
 @pytest.mark.parametrize("layout", [NHWC, NCHW])
 @pytest.mark.parametrize("combo", range(8))
 def test_epilogue_null_combinations(layout, combo):
