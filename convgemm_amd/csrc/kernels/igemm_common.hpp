@@ -127,15 +127,15 @@ __device__ inline Ctx make_ctx(const cg_kparams &p, int cls) {
 //   n: logical GEMM col (FWD/WGRAD: kn; DGRAD: c)
 // Epilogue order follows src/im2row_nhwc.c:276-282 and
 // src/im2col_nchw.c:422-431.
+// Element offset of output (m, n) in the operation's layout.
 template <int OP>
-__device__ inline void store_out(const cg_kparams &p, const Ctx &x, int m, int n, float acc) {
-    long long addr;
+__device__ inline long long out_addr(const cg_kparams &p, const Ctx &x, int m, int n) {
     if constexpr (OP == CG_FWD_NHWC) {
-        addr = (long long)m * p.KN + n;
+        return (long long)m * p.KN + n;
     } else if constexpr (OP == CG_FWD_NCHW) {
         int b = fdiv(m, x.howo);
         int pix = m - b * (p.Ho * p.Wo);
-        addr = ((long long)b * p.KN + n) * (p.Ho * p.Wo) + pix;
+        return ((long long)b * p.KN + n) * (p.Ho * p.Wo) + pix;
     } else if constexpr (OP == CG_DGRAD_NHWC || OP == CG_DGRAD_NCHW) {
         int b = fdiv(m, x.cls_hw);
         int rem = m - b * (x.Hc * x.Wc);
@@ -143,28 +143,128 @@ __device__ inline void store_out(const cg_kparams &p, const Ctx &x, int m, int n
         int j = rem - i * x.Wc;
         int ih = x.rh + p.vs * i, iw = x.rw + p.hs * j;
         if constexpr (OP == CG_DGRAD_NHWC)
-            addr = (((long long)b * p.H + ih) * p.W + iw) * p.C + n;
+            return (((long long)b * p.H + ih) * p.W + iw) * p.C + n;
         else
-            addr = (((long long)b * p.C + n) * p.H + ih) * p.W + iw;
+            return (((long long)b * p.C + n) * p.H + ih) * p.W + iw;
     } else {  // WGRAD
         int tap = fdiv(m, x.c);
         int c = m - tap * p.C;
         if constexpr (OP == CG_WGRAD_NHWC)
-            addr = ((long long)c * p.taps + tap) * p.KN + n;
+            return ((long long)c * p.taps + tap) * p.KN + n;
         else
-            addr = (long long)n * ((long long)p.C * p.taps) + (long long)c * p.taps + tap;
+            return (long long)n * ((long long)p.C * p.taps) + (long long)c * p.taps + tap;
     }
+}
+
+// The reference's epilogue order (src/im2row_nhwc.c:277-281): t = alpha*acc,
+// t += beta*out, t += bias, t = (t - mean)*inv_std, t = t*gamma + bn_beta,
+// ReLU.  `ev` holds the channel's (bias, mean, inv_std, gamma, bn_beta).
+template <int OP>
+__device__ inline float epi_value(const cg_kparams &p, float acc, float old, const float (&ev)[5]) {
     float t = p.alpha * p.scale * acc;
-    if (p.beta != 0.0f) t += p.beta * p.out[addr];
+    if (p.beta != 0.0f) t += p.beta * old;
     if constexpr (OP == CG_FWD_NHWC || OP == CG_FWD_NCHW) {
-        if (p.bias) t += p.bias[n];
+        if (p.bias) t += ev[0];
         if (p.mean) {
-            t = (t - p.mean[n]) * p.inv_std[n];
-            t = t * p.gamma[n] + p.bnb[n];
+            t = (t - ev[1]) * ev[2];
+            t = t * ev[3] + ev[4];
         }
         if (p.relu && t < 0.0f) t = 0.0f;
     }
-    p.out[addr] = t;
+    return t;
+}
+
+template <int OP>
+__device__ inline void store_out(const cg_kparams &p, const Ctx &x, int m, int n, float acc) {
+    const long long addr = out_addr<OP>(p, x, m, n);
+    float ev[5] = {0.f, 0.f, 1.f, 1.f, 0.f};
+    if constexpr (OP == CG_FWD_NHWC || OP == CG_FWD_NCHW) {
+        if (p.bias) ev[0] = p.bias[n];
+        if (p.mean) {
+            ev[1] = p.mean[n];
+            ev[2] = p.inv_std[n];
+            ev[3] = p.gamma[n];
+            ev[4] = p.bnb[n];
+        }
+    }
+    p.out[addr] = epi_value<OP>(p, acc, p.beta != 0.0f ? p.out[addr] : 0.f, ev);
+}
+
+// ---- tile write-back ----
+// A load issued after a store waits for that store as well (vmcnt counts
+// both), and the compiler cannot move loads of the epilogue vectors or of
+// beta*out across stores to `out` (they may alias).  Per-element store_out in
+// a tile's write-back therefore costs one memory round trip per output.  The
+// tile write-back instead stages the forward epilogue's per-channel vectors
+// of its columns in LDS once, and loads the beta term of 16 outputs before
+// storing any of them.
+
+// Does the tile write-back stage epilogue vectors (uniform per workgroup)?
+template <int OP>
+__device__ inline bool epi_staged(const cg_kparams &p) {
+    return (OP == CG_FWD_NHWC || OP == CG_FWD_NCHW) && (p.bias || p.mean);
+}
+
+// Stage the vectors of channels [n0, n0 + cnt) as 5 rows of cnt floats at s
+// (bias, mean, inv_std, gamma, bn_beta).  Called by all threads of the
+// workgroup between barriers.
+__device__ inline void epi_stage(const cg_kparams &p, int n0, int cnt, float *s, int tid, int nthreads) {
+    for (int i = tid; i < cnt; i += nthreads) {
+        const int n = n0 + i;
+        const bool ok = n < p.KN;
+        s[i] = (p.bias && ok) ? p.bias[n] : 0.f;
+        s[cnt + i] = (p.mean && ok) ? p.mean[n] : 0.f;
+        s[2 * cnt + i] = (p.mean && ok) ? p.inv_std[n] : 1.f;
+        s[3 * cnt + i] = (p.mean && ok) ? p.gamma[n] : 1.f;
+        s[4 * cnt + i] = (p.mean && ok) ? p.bnb[n] : 0.f;
+    }
+}
+
+// Write back one 32x32 MFMA block (16 accumulators per lane) with the fused
+// epilogue.  pos(v, m, n, nl) gives accumulator v's output (m, n), its column
+// within the staged vectors (nl), and whether the output exists.
+template <int OP, typename Pos>
+__device__ __attribute__((always_inline)) inline void store_block(const cg_kparams &p, const Ctx &x, const f32x16 &acc,
+                                                                   const float *s_epi, int cnt, Pos pos) {
+    auto vec = [&](int nl, float (&ev)[5]) __attribute__((always_inline)) {
+        ev[0] = 0.f, ev[1] = 0.f, ev[2] = 1.f, ev[3] = 1.f, ev[4] = 0.f;
+        if (s_epi) {
+#pragma unroll
+            for (int k = 0; k < 5; k++) ev[k] = s_epi[k * cnt + nl];
+        }
+    };
+    if (p.beta == 0.0f) {  // no loads at all: nothing for a store to wait behind
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            int m, n, nl;
+            if (!pos(v, m, n, nl)) continue;
+            float ev[5];
+            vec(nl, ev);
+            p.out[out_addr<OP>(p, x, m, n)] = epi_value<OP>(p, acc[v], 0.f, ev);
+        }
+        return;
+    }
+    // beta != 0: in groups of 4 outputs, the old values first, without branches
+    // (an output that does not exist reads out[0] and is not stored), then the
+    // stores -- one wait per group instead of one per output, in few registers
+#pragma unroll
+    for (int g = 0; g < 16; g += 4) {
+        float old[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            int m, n, nl;
+            const bool ok = pos(g + u, m, n, nl);
+            old[u] = p.out[ok ? out_addr<OP>(p, x, m, n) : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            int m, n, nl;
+            if (!pos(g + u, m, n, nl)) continue;
+            float ev[5];
+            vec(nl, ev);
+            p.out[out_addr<OP>(p, x, m, n)] = epi_value<OP>(p, acc[g + u], old[u], ev);
+        }
+    }
 }
 
 }  // namespace cg

@@ -614,36 +614,98 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
 
     // ---- write-back ----
     const int ncls = p.ncls_h * p.ncls_w;
+    if (piece || p.splits > 1) {  // raw partials: stream-K piece (tile-local) or split-K slice
+#pragma unroll
+        for (int a = 0; a < TR; a++)
+#pragma unroll
+            for (int b = 0; b < TC; b++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int ir = roff + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                    const int ic = coff + b * 32 + (lane & 31);
+                    const int ml = SW ? ic : ir, nl = SW ? ir : ic;
+                    if (piece) {
+                        piece[ml * BN + nl] = acc[a][b][v];
+                        continue;
+                    }
+                    int m;
+                    const int n = nt * BN + nl;
+                    bool ok;
+                    if constexpr (WG) {
+                        const int c = wg_c0 + ml;
+                        m = wg_tap * p.C + c;
+                        ok = c < p.C && n < p.KN;
+                    } else {
+                        m = mt * BM + ml;
+                        ok = m < x.Mrows && n < (DG ? p.C : p.KN);
+                    }
+                    if (ok) p.ws[(((long long)split * ncls + cls) * p.M + m) * p.N + n] = acc[a][b][v];
+                }
+        return;
+    }
+    if constexpr (!FW) {  // weight gradient, backward data: per output (batched loads cost occupancy here)
+#pragma unroll
+        for (int a = 0; a < TR; a++)
+#pragma unroll
+            for (int b = 0; b < TC; b++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int ir = roff + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                    const int ic = coff + b * 32 + (lane & 31);
+                    const int ml = SW ? ic : ir, nl = SW ? ir : ic;
+                    int m;
+                    const int n = nt * BN + nl;
+                    bool ok;
+                    if constexpr (WG) {
+                        const int c = wg_c0 + ml;
+                        m = wg_tap * p.C + c;
+                        ok = c < p.C && n < p.KN;
+                    } else {
+                        m = mt * BM + ml;
+                        ok = m < x.Mrows && n < p.C;
+                    }
+                    if (ok) store_out<OP>(p, x, m, n, acc[a][b][v]);
+                }
+        return;
+    }
+    if (!epi_staged<OP>(p) && p.beta == 0.0f) {  // plain forward: nothing to load, one store per output
+#pragma unroll
+        for (int a = 0; a < TR; a++)
+#pragma unroll
+            for (int b = 0; b < TC; b++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int ir = roff + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                    const int ic = coff + b * 32 + (lane & 31);
+                    const int ml = SW ? ic : ir, nl = SW ? ir : ic;
+                    const int m = mt * BM + ml, n = nt * BN + nl;
+                    if (m < x.Mrows && n < p.KN) store_out<OP>(p, x, m, n, acc[a][b][v]);
+                }
+        return;
+    }
+    // forward: the fused epilogue, per-channel vectors staged in LDS (the operand tiles are dead)
+    float *s_epi = nullptr;
+    if (epi_staged<OP>(p)) {
+        static_assert(5 * BN * sizeof(float) <= sizeof(sA), "epilogue vectors fit the A tile's LDS");
+        __syncthreads();
+        s_epi = reinterpret_cast<float *>(sA);
+        epi_stage(p, nt * BN, BN, s_epi, tid, NT);
+        __syncthreads();
+    }
 #pragma unroll
     for (int a = 0; a < TR; a++)
 #pragma unroll
         for (int b = 0; b < TC; b++)
-#pragma unroll
-            for (int v = 0; v < 16; v++) {
+            store_block<OP>(p, x, acc[a][b], s_epi, BN, [&](int v, int &m, int &n, int &nl) __attribute__((always_inline)) {
                 const int ir = roff + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
                 const int ic = coff + b * 32 + (lane & 31);
-                const int ml = SW ? ic : ir, nl = SW ? ir : ic;
-                if (piece) {  // stream-K partial tile, tile-local
-                    piece[ml * BN + nl] = acc[a][b][v];
-                    continue;
-                }
-                int m;
-                const int n = nt * BN + nl;
-                bool ok;
-                if constexpr (WG) {
-                    const int c = wg_c0 + ml;
-                    m = wg_tap * p.C + c;
-                    ok = c < p.C && n < p.KN;
-                } else {
-                    m = mt * BM + ml;
-                    ok = m < x.Mrows && n < (DG ? p.C : p.KN);
-                }
-                if (!ok) continue;
-                if (p.splits > 1)
-                    p.ws[(((long long)split * ncls + cls) * p.M + m) * p.N + n] = acc[a][b][v];
-                else
-                    store_out<OP>(p, x, m, n, acc[a][b][v]);
-            }
+                const int ml = SW ? ic : ir;
+                nl = SW ? ir : ic;
+                n = nt * BN + nl;
+                m = mt * BM + ml;
+                return m < x.Mrows && n < p.KN;
+            });
+    if (s_epi) __syncthreads();  // the LDS is rewritten by the next item of a persistent launch
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8);

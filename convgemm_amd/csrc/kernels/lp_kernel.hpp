@@ -347,24 +347,57 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
     }
 
     // ---- write-back (FWD NHWC: rows = output pixels, cols = kn) ----
+    if (piece || p.splits > 1) {  // raw partials: stream-K piece (tile-local) or split-K slice
+#pragma unroll
+        for (int a = 0; a < TR; a++)
+#pragma unroll
+            for (int b = 0; b < TC; b++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int ml = wm * WTM + a * 32 + 8 * (v >> 2) + 4 * h + (v & 3), nl = wn * WTN + b * 32 + l32;
+                    if (piece) {
+                        piece[ml * BN + nl] = acc[a][b][v];
+                        continue;
+                    }
+                    const int m = mt * BM + ml, n = nt * BN + nl;
+                    if (m < x.Mrows && n < p.KN) p.ws[((long long)split * p.M + m) * p.N + n] = acc[a][b][v];
+                }
+        return;
+    }
+    if (!epi_staged<CG_FWD_NHWC>(p) && p.beta == 0.0f) {  // plain: nothing to load, one store per output
+#pragma unroll
+        for (int a = 0; a < TR; a++)
+#pragma unroll
+            for (int b = 0; b < TC; b++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int ml = wm * WTM + a * 32 + 8 * (v >> 2) + 4 * h + (v & 3), nl = wn * WTN + b * 32 + l32;
+                    const int m = mt * BM + ml, n = nt * BN + nl;
+                    if (m < x.Mrows && n < p.KN) store_out<CG_FWD_NHWC>(p, x, m, n, acc[a][b][v]);
+                }
+        return;
+    }
+    // the fused epilogue: per-channel vectors staged in LDS (the pipeline stages are dead)
+    float *s_epi = nullptr;
+    if (epi_staged<CG_FWD_NHWC>(p)) {
+        static_assert(5 * BN * sizeof(float) <= STAGE, "epilogue vectors fit one LDS stage");
+        __syncthreads();
+        s_epi = reinterpret_cast<float *>(st0);
+        epi_stage(p, nt * BN, BN, s_epi, tid, NW * 64);
+        __syncthreads();
+    }
 #pragma unroll
     for (int a = 0; a < TR; a++)
 #pragma unroll
         for (int b = 0; b < TC; b++)
-#pragma unroll
-            for (int v = 0; v < 16; v++) {
-                const int ml = wm * WTM + a * 32 + 8 * (v >> 2) + 4 * h + (v & 3), nl = wn * WTN + b * 32 + l32;
-                if (piece) {  // stream-K partial tile, [BM][BN] tile-local
-                    piece[ml * BN + nl] = acc[a][b][v];
-                    continue;
-                }
-                const int m = mt * BM + ml, n = nt * BN + nl;
-                if (m >= x.Mrows || n >= p.KN) continue;
-                if (p.splits > 1)
-                    p.ws[((long long)split * p.M + m) * p.N + n] = acc[a][b][v];
-                else
-                    store_out<CG_FWD_NHWC>(p, x, m, n, acc[a][b][v]);
-            }
+            store_block<CG_FWD_NHWC>(p, x, acc[a][b], s_epi, BN, [&](int v, int &m, int &n, int &nl) __attribute__((always_inline)) {
+                const int ml = wm * WTM + a * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+                nl = wn * WTN + b * 32 + l32;
+                m = mt * BM + ml;
+                n = nt * BN + nl;
+                return m < x.Mrows && n < p.KN;
+            });
+    if (s_epi) __syncthreads();  // the LDS is rewritten by the next item of a persistent launch
 }
 
 __device__ inline int lp_xcd_remap(int bid, int nb) {
