@@ -643,29 +643,43 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
                 }
         return;
     }
-    if constexpr (!FW) {  // weight gradient, backward data: per output (batched loads cost occupancy here)
+    if constexpr (!FW) {  // weight gradient, backward data
+        auto pos = [&](int a, int b, int v, int &m, int &n) __attribute__((always_inline)) {
+            const int ir = roff + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+            const int ic = coff + b * 32 + (lane & 31);
+            const int ml = SW ? ic : ir, nl = SW ? ir : ic;
+            n = nt * BN + nl;
+            if constexpr (WG) {
+                const int c = wg_c0 + ml;
+                m = wg_tap * p.C + c;
+                return c < p.C && n < p.KN;
+            } else {
+                m = mt * BM + ml;
+                return m < x.Mrows && n < p.C;
+            }
+        };
+        if (p.beta == 0.0f) {  // nothing to load: one store per output
+#pragma unroll
+            for (int a = 0; a < TR; a++)
+#pragma unroll
+                for (int b = 0; b < TC; b++)
+#pragma unroll
+                    for (int v = 0; v < 16; v++) {
+                        int m, n;
+                        if (pos(a, b, v, m, n)) store_out<OP>(p, x, m, n, acc[a][b][v]);
+                    }
+            return;
+        }
+        // beta != 0 (backward data accumulates): old values of 4 outputs, then their
+        // stores -- one wait per 4 outputs instead of one per output
 #pragma unroll
         for (int a = 0; a < TR; a++)
 #pragma unroll
             for (int b = 0; b < TC; b++)
-#pragma unroll
-                for (int v = 0; v < 16; v++) {
-                    const int ir = roff + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-                    const int ic = coff + b * 32 + (lane & 31);
-                    const int ml = SW ? ic : ir, nl = SW ? ir : ic;
-                    int m;
-                    const int n = nt * BN + nl;
-                    bool ok;
-                    if constexpr (WG) {
-                        const int c = wg_c0 + ml;
-                        m = wg_tap * p.C + c;
-                        ok = c < p.C && n < p.KN;
-                    } else {
-                        m = mt * BM + ml;
-                        ok = m < x.Mrows && n < p.C;
-                    }
-                    if (ok) store_out<OP>(p, x, m, n, acc[a][b][v]);
-                }
+                store_block<OP>(p, x, acc[a][b], nullptr, 0, [&](int v, int &m, int &n, int &nl) __attribute__((always_inline)) {
+                    nl = 0;
+                    return pos(a, b, v, m, n);
+                });
         return;
     }
     if (!epi_staged<OP>(p) && p.beta == 0.0f) {  // plain forward: nothing to load, one store per output
