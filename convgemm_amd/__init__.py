@@ -200,6 +200,8 @@ def lib() -> ctypes.CDLL:
               "convgemm_weights_nchw_to_nhwc"):
         getattr(L, f).argtypes = [vp, vp, ci, ci, ci, ci, ci, vp]
         getattr(L, f).restype = ci
+    L.cg_dgrad_class_order.argtypes = [ctypes.POINTER(Desc), ctypes.POINTER(ci), ctypes.POINTER(ci)]
+    L.cg_dgrad_class_order.restype = ci
     L.cg_split_factors.argtypes = [ctypes.POINTER(ci), ci]
     L.cg_split_factors.restype = ci
     L.cg_num_tilecfgs.argtypes = [ci]
@@ -634,6 +636,14 @@ def tile_configs(dtype=F32):
         if L.cg_get_tilecfg(int(dtype), c, ctypes.byref(v)) == 0:
             out.append(tuple(v))
     return out
+
+
+def dgrad_class_order(desc: Desc):
+    """Backward-data launch order of the stride residue classes (class = rh * hstride + rw) and
+    each class's tap count; ([], []) when the order is the identity."""
+    o, t = (ctypes.c_int * 16)(), (ctypes.c_int * 16)()
+    n = lib().cg_dgrad_class_order(ctypes.byref(desc), o, t)
+    return [o[i] for i in range(n)], [t[i] for i in range(n)]
 
 
 def split_factors():

@@ -426,6 +426,25 @@ int cg_weight_repack(const convgemm_desc_t *d, int elem, int *d0, int *d1, int *
     return weight_repack(d, elem, d0, d1, d2, perm);
 }
 
+/* Test hook: backward-data launch order of the residue classes of a problem
+ * (order[i] = the i-th class launched, class = rh * hs + rw) and each class's
+ * tap count; returns the number of classes, or 0 when the order is the
+ * identity (one class, or more than 16). */
+int cg_dgrad_class_order(const convgemm_desc_t *d, int *order, int *taps) {
+    cg_kparams kp;
+    memset(&kp, 0, sizeof(kp));
+    kp.KH = d->kh, kp.KW = d->kw, kp.vp = d->vpad, kp.hp = d->hpad;
+    kp.vs = d->vstride, kp.hs = d->hstride, kp.vd = d->vdil, kp.hd = d->hdil;
+    const unsigned long long ord = dgrad_class_order(&kp);
+    if (!ord) return 0;
+    const int n = kp.vs * kp.hs;
+    for (int i = 0; i < n; i++) {
+        order[i] = (int)((ord >> (4 * i)) & 15);
+        taps[i] = class_taps(i / kp.hs, kp.vp, kp.KH, kp.vd, kp.vs) * class_taps(i % kp.hs, kp.hp, kp.KW, kp.hd, kp.hs);
+    }
+    return n;
+}
+
 static int vec_ok(int elem, const convgemm_desc_t *d, const void *a, const void *b) {
     int vw = 16 / esize_of(elem);
     if (((uintptr_t)a % 16) || ((uintptr_t)b % 16)) return 0;

@@ -254,3 +254,34 @@ def test_tuning_database_holds_full_candidate_lists(tmp_path):
     assert len(got) == n
     assert [(c, s) for c, s, _ in got] == [(i % 31, 1 + i % 200) for i in range(n)]
     assert all(abs(g - u) <= 1e-3 * u for (_, _, g), u in zip(got, us))
+
+
+def _taps_brute(r, pad, k, dil, s):
+    # input rows i = r + s*t receive kernel tap kx when i + pad - kx*dil is a multiple of s
+    return sum(1 for kx in range(k) if (r + pad - kx * dil) % s == 0)
+
+
+@pytest.mark.parametrize("k,pad,st,dil", [(3, 1, 2, 1), (1, 0, 2, 1), (5, 2, 2, 1), (3, 1, 3, 1), (3, 0, 2, 2),
+                                          (4, 1, 2, 1), (7, 3, 4, 1)])
+def test_dgrad_class_order_most_taps_first(k, pad, st, dil):
+    """Backward data launches the stride residue classes with the most kernel taps first
+    (a stable order: ties keep class order); every class appears once."""
+    d = cg.make_desc(cg.DGRAD, cg.NHWC, 2, 16, 16, 8, 8, k, k, pad, pad, st, st, dil, dil)
+    order, taps = cg.dgrad_class_order(d)
+    n = st * st
+    if n > 16:
+        assert order == []
+        return
+    assert sorted(order) == list(range(n))
+    want = [_taps_brute(c // st, pad, k, dil, st) * _taps_brute(c % st, pad, k, dil, st) for c in range(n)]
+    assert taps == want
+    assert [taps[c] for c in order] == sorted(want, reverse=True)
+    for i in range(1, n):  # stable among equal tap counts
+        if taps[order[i]] == taps[order[i - 1]]:
+            assert order[i] > order[i - 1]
+
+
+def test_dgrad_class_order_3x3_stride2():
+    """3x3, stride 2, pad 1: classes (0,0) 1 tap, (0,1) and (1,0) 2 taps, (1,1) 4 taps."""
+    d = cg.make_desc(cg.DGRAD, cg.NHWC, 1, 8, 8, 4, 4, 3, 3, 1, 1, 2, 2)
+    assert cg.dgrad_class_order(d) == ([3, 1, 2, 0], [1, 2, 2, 4])
