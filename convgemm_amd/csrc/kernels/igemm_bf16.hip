@@ -38,12 +38,13 @@ const cg_tilecfg kCfg[] = {
     {512, 64, 64, 8, 1, 2},  // 27 pipelined, 8 waves, 2 LDS stages
     {256, 64, 64, 4, 2, 3},  // 28 pipelined, 8 waves, 3 LDS stages
     {128, 128, 64, 2, 4, 4},  // 29 pipelined, 8 waves, 4 LDS stages
+    {64, 64, 128, 2, 2},  // 30 long K step (weight gradient: fewer barriers per MFMA)
 };
 constexpr int kNum = sizeof(kCfg) / sizeof(kCfg[0]);
 // configurations that also have element-load (non-16-byte) instantiations
-const bool kNoVec[kNum] = {true, false, false, true, false, false, false, true, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false};
+const bool kNoVec[kNum] = {true, false, false, true, false, false, false, true, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false};
 // configurations served by the pipelined forward kernel (lp_kernel.hpp)
-const bool kLp[kNum] = {false, false, false, false, false, false, false, false, false, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true};
+const bool kLp[kNum] = {false, false, false, false, false, false, false, false, false, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, false};
 
 }  // namespace
 
@@ -90,6 +91,7 @@ static int launch_any(int cfg, const cg_kparams *kp, const cg_group *g, int *occ
     case 27: return launch_lp<uint16_t, 512, 64, 8, 1, 2>(kp, g, occ, st);
     case 28: return launch_lp<uint16_t, 256, 64, 4, 2, 3>(kp, g, occ, st);
     case 29: return launch_lp<uint16_t, 128, 128, 2, 4, 4>(kp, g, occ, st);
+    case 30: return launch_cfg<uint16_t, 64, 64, 128, 2, 2, false>(kp, g, occ, st);
     }
     return (int)hipErrorInvalidValue;
 }

@@ -40,20 +40,21 @@ const cg_tilecfg kCfg[] = {
     {256, 128, 32, 4, 2, 2},  // 28 pipelined, 8 waves, 2 LDS stages
     {128, 128, 32, 2, 4, 3},  // 29 pipelined, 8 waves, 3 LDS stages
     {256, 64, 32, 4, 2, 3},  // 30 pipelined, 8 waves, 3 LDS stages
+    {64, 64, 64, 2, 2},  // 31 long K step (weight gradient: fewer barriers per MFMA)
 };
 constexpr int kNum = sizeof(kCfg) / sizeof(kCfg[0]);
 // configurations that also have element-load (non-16-byte) instantiations
 const bool kNoVec[kNum] = {true, false, false, true, false, false, true, false, false, false, false, false,
                            false, false, false, false, false, false, false, false, false, false,
-                           false, false, false, false, false, false, false, false, false};
+                           false, false, false, false, false, false, false, false, false, false};
 // configurations served by the pipelined forward kernel (lp_kernel.hpp)
 const bool kLp[kNum] = {false, false, false, false, false, false, false, false, false, false, false, false,
                         true, true, true, true, true, true, true, true, true, true,
-                        false, false, false, false, false, false, true, true, true};
+                        false, false, false, false, false, false, true, true, true, false};
 // configurations served by the pipelined forward NCHW kernel (lp_nchw_kernel.hpp)
 const bool kLpn[kNum] = {false, false, false, false, false, false, false, false, false, false, false, false,
                          false, false, false, false, false, false, false, false, false, false,
-                         true, true, true, true, true, true, false, false, false};
+                         true, true, true, true, true, true, false, false, false, false};
 
 }  // namespace
 
@@ -101,6 +102,7 @@ static int launch_any(int cfg, const cg_kparams *kp, const cg_group *g, int *occ
     case 28: return launch_lp<float, 256, 128, 4, 2, 2>(kp, g, occ, st);
     case 29: return launch_lp<float, 128, 128, 2, 4, 3>(kp, g, occ, st);
     case 30: return launch_lp<float, 256, 64, 4, 2, 3>(kp, g, occ, st);
+    case 31: return launch_cfg<float, 64, 64, 64, 2, 2, false>(kp, g, occ, st);
     }
     return (int)hipErrorInvalidValue;
 }

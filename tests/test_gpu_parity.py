@@ -97,8 +97,12 @@ def test_backward_data_accumulates(layout, si):
 CFG_SHAPES = [Shape(2, 11, 10, 24, 40, 3, 3, 1, 1, 1, 1, 1, 1), Shape(1, 13, 12, 20, 36, 3, 3, 1, 0, 2, 2, 1, 1)]
 
 
+# every configuration of the general kernel (the pipelined ones have tests of their own)
+GENERAL_CFGS = [c for c, t in enumerate(cg.tile_configs(cg.F32)) if t[5] == 0]
+
+
 @pytest.mark.parametrize("splits", [0, 1, 3])
-@pytest.mark.parametrize("cfg", range(12))
+@pytest.mark.parametrize("cfg", GENERAL_CFGS)
 def test_every_tile_configuration(cfg, splits):
     """Force each kernel instantiation (and the split-K reduction; splits 0 =
     stream-K with its fix-up, forward and weight gradient) on every op.
