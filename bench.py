@@ -332,43 +332,184 @@ def batch_sweep(torch, cg, stream, n=256, reps=10, seed=5):
         cg.set_planner(cg.PLAN_TUNED)
 
 
-def cpu_baseline(budget_s=12.0):
-    """Time the oracle's FP32 restatement of the reference's implicit GEMM
-    (oracle/convgemm_port.c) on the host cores, on the same layer sweep at
-    batch 1 (1/32 of a step), repeated until ~budget_s seconds have passed."""
-    import oracle
+def explicit_vs_implicit(torch, cg, stream, reps=10, batch=32):
+    """The reference drivers' ablation (tests/test_base.c:45-88) on the GPU, per
+    layer at batch 32, FP32 and BF16 operands: the explicit path -- zero-fill
+    the patch matrix, im2row (NHWC) / im2col (NCHW) into it, a plain GEMM on
+    it (the same MFMA kernels, run as a 1x1 convolution over the patch
+    matrix), the bias / batch-norm / ReLU epilogue as separate
+    elementwise passes (torch, like the reference's separate loops), and for
+    NCHW the transpose to [b][kn][ho][wo] -- against the implicit call with
+    the fused epilogue.  Every stage is timed alone with HIP events over
+    `reps` back-to-back launches; the explicit total is the sum of its
+    stages.  Both results are compared (max|explicit - implicit| / max|.|)."""
+    res = {}
+    for li, (name, spec) in enumerate(LAYERS):
+        b, h, w, c, kn, kh, kw, pad, st = spec
+        ho, wo = out_hw(h, pad, kh, st), out_hw(w, pad, kw, st)
+        P, K = batch * ho * wo, c * kh * kw
+        flops = layer_flops((batch,) + spec[1:])
+        for layout in (0, 1):
+            d = make_inputs(torch, layout, (batch,) + spec[1:], 0xC0FFEE + 6000 + 10 * li + layout)
+            desc = cg.make_desc(cg.FWD, layout, batch, h, w, c, kn, kh, kw, pad, pad, st, st)
+            for dt, dname in ((cg.F32, "fp32"), (cg.BF16, "bf16")):
+                if dt == cg.F32:
+                    X, W, tdt = d["X"], d["W"], torch.float32
+                else:
+                    X = torch.empty(d["X"].shape, dtype=torch.int16, device="cuda")
+                    W = torch.empty(d["W"].shape, dtype=torch.int16, device="cuda")
+                    cg.quantize(d["X"], X, cg.BF16, stream=stream)
+                    cg.quantize(d["W"], W, cg.BF16, stream=stream)
+                    tdt = torch.int16
+                pm = torch.empty(P * K, dtype=tdt, device="cuda")  # patch matrix: rows P x K / cols K x P
+                Yi = torch.empty(d["Y"].shape, device="cuda")
+                Ye = torch.empty(d["Y"].shape, device="cuda")
+                Yg = torch.empty(kn * P, device="cuda") if layout == 1 else Ye
+                vec = (lambda v: v.view(1, kn, 1, 1)) if layout == 1 else (lambda v: v)
+                bias, mean, inv_std = vec(d["bias"]), vec(d["mean"]), vec(d["inv_std"])
+                gamma, bnb = vec(d["gamma"]), vec(d["bnbeta"])
+
+                def zero():
+                    pm.zero_()
+
+                def transform():
+                    cg.im2row_ex(desc, X, pm, K if layout == 0 else P, dtype=dt, stream=stream)
+
+                # the plain GEMM on the patch matrix, as a 1x1 convolution on the same MFMA kernels:
+                # NHWC Y (P x kn) = rows (P x K) W (K x kn); NCHW Yg (kn x P) = W (kn x K) cols (K x P)
+                gdesc = (cg.make_desc(cg.FWD, 0, P, 1, 1, K, kn, 1, 1) if layout == 0
+                         else cg.make_desc(cg.FWD, 1, 1, 1, P, K, kn, 1, 1))
+
+                def gemm():
+                    cg.conv_ex(gdesc, W, pm, Ye if layout == 0 else Yg, dtype=dt, stream=stream)
+
+                def transpose():
+                    cg.transpose_nchw_ex(P, kn, Yg, P, 0.0, Ye, kn, ho, wo, stream=stream)
+
+                def epilogue():
+                    Ye.add_(bias)
+                    Ye.sub_(mean).mul_(inv_std).mul_(gamma).add_(bnb)
+                    Ye.clamp_(min=0.0)
+
+                def implicit():
+                    cg.conv_ex(desc, W, X, Yi, bias=d["bias"], mean=d["mean"], inv_std=d["inv_std"],
+                               gamma=d["gamma"], bnbeta=d["bnbeta"], relu=True, dtype=dt, stream=stream)
+
+                def timed(fn):
+                    fn()
+                    torch.cuda.synchronize()
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(stream)
+                    for _ in range(reps):
+                        fn()
+                    e1.record(stream)
+                    torch.cuda.synchronize()
+                    return e0.elapsed_time(e1) / reps
+
+                with torch.cuda.stream(stream):
+                    stages = {"zero_fill": timed(zero), "im2row" if layout == 0 else "im2col": timed(transform),
+                              "gemm": timed(gemm)}
+                    if layout == 1:
+                        stages["transpose_nchw"] = timed(transpose)
+                    stages["epilogue_passes"] = timed(epilogue)
+                    t_imp = timed(implicit)
+                    # one clean explicit evaluation, compared with the implicit result
+                    zero(); transform(); gemm()
+                    if layout == 1:
+                        transpose()
+                    epilogue()
+                    torch.cuda.synchronize()
+                diff = float((Ye - Yi).abs().max() / Yi.abs().max().clamp_min(1e-30))
+                t_exp = sum(stages.values())
+                res[f"{name}_{'NHWC' if layout == 0 else 'NCHW'}_{dname}"] = {
+                    "implicit": {"ms": round(t_imp, 4), "tflops": round(flops / (t_imp * 1e-3) / 1e12, 2)},
+                    "explicit": {"ms": round(t_exp, 4), "tflops": round(flops / (t_exp * 1e-3) / 1e12, 2),
+                                 "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+                                 "patch_matrix_bytes": P * K * (4 if dt == cg.F32 else 2)},
+                    "speedup_implicit": round(t_exp / t_imp, 2),
+                    "transform_share_of_explicit": round(
+                        (stages["zero_fill"] + stages.get("im2row", 0) + stages.get("im2col", 0)
+                         + stages.get("transpose_nchw", 0)) / t_exp, 3),
+                    "max_rel_diff_explicit_vs_implicit": diff}
+                del pm, Yi, Ye, Yg
+    return res
+
+
+def _cpu_problems(batch=32):
+    """The bench step's ten forward convolutions at `batch` on the host (FP32, seeded)."""
     from oracle import Shape
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-    threads = max(1, min(threads, len(os.sched_getaffinity(0))))
-    oracle.port_set_threads(threads)
     rng = np.random.default_rng(7)
     probs = []
     for layout in (0, 1):
         for name, spec in LAYERS:
             _, h, w, c, kn, kh, kw, pad, st = spec
-            s = Shape(1, h, w, c, kn, kh, kw, pad, pad, st, st, 1, 1)
-            X = rng.standard_normal(s.x_shape(layout)).astype(np.float32)
-            W = (rng.standard_normal(s.w_shape(layout)) * np.sqrt(2.0 / s.K)).astype(np.float32)
-            v = lambda: rng.standard_normal(kn).astype(np.float32)
-            epi = dict(bias=v(), mean=v(), inv_std=np.abs(v()) + 0.5, gamma=np.abs(v()) + 0.5, bnbeta=v(), relu=True)
-            out = np.empty(s.y_shape(layout), np.float32)
-            probs.append((layout, s, W, X, epi, out))
-    flops_per_rep = sum(p[1].flops() for p in probs)
-    for p in probs:  # warm-up
-        oracle.port_conv(p[0], "N", p[1], p[2], p[3], out=p[5], **p[4])
-    reps = 0
-    t0 = time.perf_counter()
-    while True:
-        for p in probs:
-            oracle.port_conv(p[0], "N", p[1], p[2], p[3], out=p[5], **p[4])
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or reps >= 100000:
-            break
-    return {"value": flops_per_rep * reps / el / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
-            "sample": f"same 10-convolution sweep at batch 1 (1/32 of a step), {reps} reps in {el:.1f} s, "
-                      f"FP32 restatement of the reference's B3A2C0 implicit GEMM (portable micro-kernel, "
-                      f"OpenMP {threads} threads)"}
+            s = Shape(batch, h, w, c, kn, kh, kw, pad, pad, st, st, 1, 1)
+            X = rng.standard_normal(s.x_shape(layout), dtype=np.float32)
+            W = (rng.standard_normal(s.w_shape(layout), dtype=np.float32) * np.float32(np.sqrt(2.0 / s.K)))
+            v = lambda: rng.standard_normal(kn, dtype=np.float32)  # noqa: E731
+            epi = dict(bias=0.1 * v(), mean=0.1 * v(), inv_std=(1.0 / np.sqrt(rng.uniform(0.5, 2.0, kn) + 1e-5)).astype(
+                np.float32), gamma=rng.uniform(0.5, 1.5, kn).astype(np.float32), bnbeta=0.1 * v(), relu=True)
+            probs.append((layout, s, W, X, epi, np.empty(s.y_shape(layout), np.float32)))
+    return probs
+
+
+def cpu_worker(reps):
+    """One process of the CPU baseline: 1 warm-up step, then `reps` timed steps of the
+    oracle's FP32 restatement of the reference's implicit GEMM; prints the step times."""
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    threads = max(1, min(threads, len(os.sched_getaffinity(0))))
+    oracle.port_set_threads(threads)
+    probs = _cpu_problems()
+    times = []
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
+        for layout, s, W, X, epi, out in probs:
+            oracle.port_conv(layout, "N", s, W, X, out=out, **epi)
+        if r > 0:
+            times.append(time.perf_counter() - t0)
+    print(json.dumps({"step_seconds": times, "threads": threads}), flush=True)
+    return 0
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(procs=5, reps=5):
+    """The bench step (the same ten batch-32 forward convolutions with the fused
+    epilogue) on the host cores through the oracle's FP32 restatement of the
+    reference's B3A2C0 implicit GEMM (oracle/convgemm_port.c: OpenMP, portable
+    micro-kernel; the reference itself needs BLIS, which this image lacks).
+    Protocol of the reference's drivers (tests/CMakeLists.txt:20,
+    tests/test_base.c:44): `procs` separate processes, each 1 warm-up and
+    `reps` timed steps; value = step FLOPs / median step time."""
+    import subprocess
+    times, threads = [], None
+    for _ in range(procs):
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker", str(reps)],
+                           stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"cpu baseline worker failed: {r.stderr[-2000:]}")
+        rec = json.loads(r.stdout.strip().splitlines()[-1])
+        times += rec["step_seconds"]
+        threads = rec["threads"]
+    step_flops = sum(layer_flops(spec) for _, spec in LAYERS) * 2
+    med = float(np.median(times))
+    return {"value": step_flops / med / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "median_step_s": round(med, 4), "step_s_min_max": [round(min(times), 4), round(max(times), 4)],
+            "sample": f"the full bench step (10 forward convolutions, batch 32, fused bias+BN+ReLU): "
+                      f"{procs} processes x (1 warm-up + {reps} timed steps), median of {len(times)}; FP32 "
+                      f"restatement of the reference's B3A2C0 implicit GEMM (portable micro-kernel, OpenMP "
+                      f"{threads} threads)"}
 
 
 def copy_probe(torch, mib, steps, warmup):
@@ -441,9 +582,14 @@ def main(argv=None):
                     help="only run a plain device copy of MIB MiB per step (calibrates HBM byte counters)")
     ap.add_argument("--backward-only", action="store_true",
                     help="only run the backward sweep and print it (for profiling its kernels)")
+    ap.add_argument("--no-ablation", action="store_true", help="skip the explicit-vs-implicit comparison")
+    ap.add_argument("--cpu-worker", type=int, default=0, metavar="REPS",
+                    help="internal: run one CPU-baseline process with REPS timed steps and exit")
     ap.add_argument("--only", default="", help="comma-separated layer names (e.g. L2_3x3_64to64_56_NHWC) "
                     "to run alone, for profiling; the JSON value then covers only those")
     a = ap.parse_args(argv)
+    if a.cpu_worker:  # a host-only child process: never touches the GPU
+        return cpu_worker(a.cpu_worker)
     if a.gpus != 1:
         print(f"bench.py: this build targets one GPU; --gpus {a.gpus} is not supported "
               "(sharding the convolution by batch across GPUs is a design, not part of this build)", file=sys.stderr)
@@ -576,11 +722,17 @@ def main(argv=None):
     # passes scaled by a plain copy (profiles/traffic_fwd_fp32.md), valid while
     # the layer runs the plan that was measured
     traffic = None
+    traffic_source = "no committed counter figure for this kernel"
     tpath = os.path.join(ROOT, "profiles", "traffic_fwd_fp32.json")
     if os.path.exists(tpath):
         rec = json.load(open(tpath))["layers"].get(dom["name"])
         if rec and (rec["plan"]["cfg"], rec["plan"]["splits"]) == tuple(dom["plan"][:2]):
             traffic = rec["traffic_bytes"]
+            traffic_source = ("profiles/traffic_fwd_fp32.json: rocprofv3 FETCH_SIZE and WRITE_SIZE of this "
+                              "kernel's plan, scaled by a plain copy's readings (committed, not measured live)")
+        elif rec:
+            traffic_source = (f"the committed counter figure is for plan {rec['plan']}, this run used "
+                              f"{list(dom['plan'][:2])}")
 
     errors = {}
     if not a.no_error:
@@ -612,7 +764,7 @@ def main(argv=None):
         "roofline": {"bound": "mfma", "kernel": dom["name"], "kernel_us": round(kernel_ms * 1e3, 2),
                      "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": traffic},
+                     "traffic": traffic, "traffic_source": traffic_source},
         "fp64_max_rel_err": max(errors.values()) if errors else None,
         "layers": layers,
     }
@@ -622,6 +774,8 @@ def main(argv=None):
         out["backward_sweep"] = backward_sweep(torch, cg, stream)
     if not a.no_batch:
         out["batch_sweep"] = batch_sweep(torch, cg, stream)
+    if not a.no_ablation:
+        out["explicit_vs_implicit"] = explicit_vs_implicit(torch, cg, stream)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

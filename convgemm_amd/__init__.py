@@ -26,7 +26,9 @@ __all__ = [
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libconvgemm.so")
+# CONVGEMM_LIB names another build of the same library (the sanitizer test runs the
+# host-layer checks against an ASan/UBSan build this way)
+LIB_PATH = os.environ.get("CONVGEMM_LIB") or os.path.join(_HERE, "libconvgemm.so")
 
 FWD, WGRAD, DGRAD = 0, 1, 2
 NHWC, NCHW = 0, 1
@@ -52,6 +54,7 @@ EXPORTED_SYMBOLS = (
     "convgemm_batch_create", "convgemm_batch_run", "convgemm_batch_info", "convgemm_batch_destroy",
     # convgemm_explicit.h
     "im2row_nhwc", "row2im_nhwc", "im2col_nchw", "col2im_nchw", "transpose_nchw", "convgemm_gemm",
+    "convgemm_im2row_ex", "convgemm_transpose_nchw_ex",
     "convgemm_nchw_to_nhwc", "convgemm_nhwc_to_nchw", "convgemm_weights_nhwc_to_nchw",
     "convgemm_weights_nchw_to_nhwc",
 )
@@ -216,6 +219,12 @@ def lib() -> ctypes.CDLL:
     L.cg_last_plan.restype = None
     L.cg_debug_fail_next_allocs.argtypes = [ci]
     L.cg_debug_fail_next_allocs.restype = None
+    L.convgemm_im2row_ex.argtypes = [ctypes.POINTER(Desc), ci, vp, vp, ctypes.c_longlong, vp]
+    L.convgemm_im2row_ex.restype = ci
+    L.convgemm_transpose_nchw_ex.argtypes = [ci, ci, vp, ctypes.c_longlong, cf, vp, ci, ci, ci, ci, ci, vp]
+    L.convgemm_transpose_nchw_ex.restype = ci
+    L.cg_set_chunk_window.argtypes = [ctypes.c_longlong]
+    L.cg_set_chunk_window.restype = None
     _lib = L
     return L
 
@@ -330,6 +339,30 @@ def conv_ex(desc: Desc, in_, x, out, alpha=1.0, beta=0.0, bias=None, mean=None, 
                             ctypes.byref(epi), int(dtype), sc, _ptr(workspace), int(ws_bytes), sh)
     if st != 0:
         raise ConvGemmError(st, L.convgemm_last_error_message().decode())
+
+
+def im2row_ex(desc: Desc, x, out, ld, dtype=F32, stream=None) -> None:
+    """convgemm_im2row_ex: the patch matrix of desc's input (NHWC rows [P][ld] / NCHW cols [K][ld]) in
+    FP32 or BF16 elements, asynchronous on `stream`; only in-bounds taps are written."""
+    L = lib()
+    st = L.convgemm_im2row_ex(ctypes.byref(desc), int(dtype), _ptr(x), _ptr(out), int(ld), _stream_handle(stream))
+    if st != 0:
+        raise ConvGemmError(st, L.convgemm_last_error_message().decode())
+
+
+def transpose_nchw_ex(rows, cols, in_, ld, beta, out, kn, ho, wo, start_row=0, start_col=0, stream=None) -> None:
+    """convgemm_transpose_nchw_ex: transpose_nchw asynchronous on `stream`."""
+    L = lib()
+    st = L.convgemm_transpose_nchw_ex(rows, cols, _ptr(in_), int(ld), float(beta), _ptr(out), kn, ho, wo, start_row,
+                                      start_col, _stream_handle(stream))
+    if st != 0:
+        raise ConvGemmError(st, L.convgemm_last_error_message().decode())
+
+
+def set_chunk_window(nbytes: int) -> None:
+    """Test hook: bytes one batch chunk may address (0 restores 2 GiB); smaller values make small
+    problems run the batch-chunking path."""
+    lib().cg_set_chunk_window(int(nbytes))
 
 
 def quantize(src, dst, dtype, stream=None) -> float:

@@ -1,6 +1,6 @@
 /*
  * cg_internal.h -- the contract between the C host layer (host/convgemm.c)
- * and the HIP launchers (kernels/*.hip).  Plain C: no HIP or C++ types.
+ * and the HIP launchers (the .hip files under kernels/).  Plain C: no HIP or C++ types.
  *
  * The six convolution operations are all expressed as one implicit GEMM
  * C[M][N] = sum_k A[M][k] * B[k][N] whose operands are gathered from the
@@ -17,6 +17,8 @@
  */
 #ifndef CONVGEMM_AMD_CG_INTERNAL_H
 #define CONVGEMM_AMD_CG_INTERNAL_H
+
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -117,8 +119,9 @@ typedef struct {
 } cg_tshape;
 /* nchw = 0: rows[p*ld + r] = X[b][ix][iy][c] (im2row);  nchw = 1:
  * cols[r*ld + p] = X[b][c][ix][iy] (im2col); only in-bounds taps written.
+ * esize: element size, 4 (FP32) or 2 (BF16).
  * p = (b*ho + x)*wo + y, r = (c*kh + kx)*kw + ky. */
-int cg_launch_im2row(const cg_tshape *s, const float *x, float *out, long long ld, int nchw, void *stream);
+int cg_launch_im2row(const cg_tshape *s, const void *x, void *out, long long ld, int nchw, int esize, void *stream);
 /* X += row2im(rows) (nchw = 0) or col2im(cols) (nchw = 1). */
 int cg_launch_row2im(const cg_tshape *s, const float *in, long long ld, float *x, int nchw, void *stream);
 int cg_launch_transpose_nchw(int rows, int cols, const float *in, long long ld, float beta, float *out, int kn,

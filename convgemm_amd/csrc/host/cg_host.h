@@ -58,4 +58,23 @@ size_t cg_sk_bytes(int groups, const cg_tilecfg *t);
 /* The library stream of the drop-in calls (hipStream_t). */
 void *cg_get_stream(void);
 
+/* Is `stream` (hipStream_t) being captured into a graph? */
+int cg_stream_capturing(void *stream);
+
+/* A grow-only internal device buffer (see cg_growbuf_get in convgemm.c). */
+typedef struct {
+    void *ptr;
+    size_t bytes;
+    int retired;      /* outgrown buffers kept allocated */
+    const char *what; /* name used in error messages */
+} cg_growbuf;
+/* At least `need` bytes of gb for work queued on `stream`; never frees a
+ * buffer, and fails with CONVGEMM_ERR_WORKSPACE instead of growing while the
+ * stream is being captured. */
+convgemm_status_t cg_growbuf_get(cg_growbuf *gb, size_t need, void *stream, void **out);
+
+/* Test hook: bytes one batch chunk may address (0 restores 2 GiB).  Smaller
+ * values make small problems run the batch-chunking path. */
+void cg_set_chunk_window(long long bytes);
+
 #endif

@@ -36,9 +36,6 @@ static _Thread_local int t_plan_cfg = -1, t_plan_splits = 0, t_plan_kernels = 0;
 static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
 static hipStream_t g_stream = NULL;
 
-/* internal workspace used when the caller passes none */
-static void *g_ws = NULL;
-static size_t g_ws_bytes = 0;
 /* device scratch of the quantizer (amax bits, scale) */
 static void *g_qscratch = NULL;
 
@@ -54,6 +51,10 @@ static cg_plan_hook_t g_plan_hook = NULL;
 static void *g_plan_hook_ctx = NULL;
 
 static int g_fail_allocs = 0;
+
+/* bytes one batch chunk may address (test hook; 0: CG_WINDOW) */
+static long long g_chunk_window = 0;
+void cg_set_chunk_window(long long bytes) { g_chunk_window = bytes > 0 && bytes < CG_WINDOW ? bytes : 0; }
 
 static convgemm_status_t set_error(convgemm_status_t st, const char *fmt, ...) {
     t_last_status = st;
@@ -184,6 +185,9 @@ static convgemm_status_t validate(const convgemm_desc_t *d, int *ho, int *wo) {
 
 /* --------------------------------------------------------- workspace */
 
+/* Size of a buffer handed out by alloc_pack_buffs, or 0 for any other pointer
+ * (the library does not know how large a caller's own buffer is, so it never
+ * writes to one it did not allocate; such a call uses the internal workspace). */
 static size_t pack_bytes_of(const void *p) {
     if (!p) return 0;
     size_t r = 0;
@@ -191,32 +195,53 @@ static size_t pack_bytes_of(const void *p) {
     for (int i = 0; i < CG_MAX_PACKS; i++)
         if (g_packs[i].ptr == p) r = g_packs[i].bytes;
     pthread_mutex_unlock(&g_lock);
-    /* unknown caller buffer: the reference contract says it is at least as
-     * large as what alloc_pack_buffs hands out */
-    return r ? r : (size_t)CG_PACK_BYTES;
+    return r;
 }
 
-static convgemm_status_t internal_ws(size_t need, void **out) {
-    pthread_mutex_lock(&g_lock);
-    if (need > g_ws_bytes) {
-        /* the old buffer may still be read by queued work on any stream */
-        if (g_ws) {
-            hipDeviceSynchronize();
-            hipFree(g_ws);
-        }
-        g_ws = NULL;
-        g_ws_bytes = 0;
-        size_t sz = need + (need >> 2);
-        if (lib_malloc(&g_ws, sz) != hipSuccess) {
-            g_ws = NULL;
-            pthread_mutex_unlock(&g_lock);
-            return set_error(CONVGEMM_ERR_ALLOC, "cannot allocate %zu bytes of internal workspace", sz);
-        }
-        g_ws_bytes = sz;
+int cg_stream_capturing(void *stream) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing((hipStream_t)stream, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
     }
-    *out = g_ws;
+    return cs != hipStreamCaptureStatusNone;
+}
+
+/* Grow-only device buffer shared by the calls that bring no workspace.
+ * A buffer that is outgrown is retired, never freed: queued work, or a graph
+ * captured earlier, may still use it (retired buffers stay allocated until
+ * the process ends; with growth by 1.25x their total is at most 4x the live
+ * one).  While `stream` is being captured the buffer does not grow: the call
+ * fails with CONVGEMM_ERR_WORKSPACE and the caller passes a workspace. */
+convgemm_status_t cg_growbuf_get(cg_growbuf *gb, size_t need, void *stream, void **out) {
+    pthread_mutex_lock(&g_lock);
+    if (need > gb->bytes) {
+        if (cg_stream_capturing(stream)) {
+            pthread_mutex_unlock(&g_lock);
+            return set_error(CONVGEMM_ERR_WORKSPACE,
+                             "the stream is being captured and the internal %s (%zu bytes) would have to grow to "
+                             "%zu bytes: pass a workspace (convgemm_workspace_size bytes) to calls made under capture",
+                             gb->what, gb->bytes, need);
+        }
+        void *nb = NULL;
+        size_t sz = need + (need >> 2);
+        if (lib_malloc(&nb, sz) != hipSuccess) {
+            pthread_mutex_unlock(&g_lock);
+            return set_error(CONVGEMM_ERR_ALLOC, "cannot allocate %zu bytes of %s", sz, gb->what);
+        }
+        if (gb->ptr) gb->retired++; /* the old buffer stays allocated (see above) */
+        gb->ptr = nb;
+        gb->bytes = sz;
+    }
+    *out = gb->ptr;
     pthread_mutex_unlock(&g_lock);
     return CONVGEMM_SUCCESS;
+}
+
+static cg_growbuf g_ws = {NULL, 0, 0, "workspace"};
+
+static convgemm_status_t internal_ws(size_t need, hipStream_t stream, void **out) {
+    return cg_growbuf_get(&g_ws, need, stream, out);
 }
 
 int alloc_pack_buffs(float **Ac_pack, float **Bc_pack) {
@@ -497,7 +522,8 @@ convgemm_status_t cg_prepare(const convgemm_desc_t *d, convgemm_dtype_t dtype, c
     image_bytes(d, ho, wo, es, &img_a, &img_o, &img_b);
     long long img_max = img_a > img_o ? img_a : img_o;
     if (img_b > img_max) img_max = img_b;
-    int chunk = (int)((CG_WINDOW - 1) / (img_max > 0 ? img_max : 1));
+    const long long window = g_chunk_window > 0 ? g_chunk_window : CG_WINDOW;
+    int chunk = (int)((window - 1) / (img_max > 0 ? img_max : 1));
     if (chunk > d->b) chunk = d->b;
     if (chunk < 1) chunk = 1;
     kp->B = chunk;
@@ -618,12 +644,9 @@ static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, 
 
     /* plan on the first chunk's shape */
     int cfg, splits;
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(stream, &cs) != hipSuccess) {
-        (void)hipGetLastError();
-        cs = hipStreamCaptureStatusNone;
-    }
-    t_in_capture = cs != hipStreamCaptureStatusNone;
+    /* under capture the planner may not run device work (the cost model's
+     * inference allocates and copies) */
+    t_in_capture = cg_stream_capturing(stream);
     choose_plan(elem, &kp, &cfg, &splits);
     t_in_capture = 0;
     cg_tilecfg t;
@@ -646,7 +669,7 @@ static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, 
     if ((repack && !ws->ws_w) || (pbytes && !ws->ws_parts)) {
         size_t wt_al = (wt_bytes + 255) & ~(size_t)255;
         void *base = NULL;
-        st = internal_ws(wt_al + pbytes, &base);
+        st = internal_ws(wt_al + pbytes, stream, &base);
         if (st != CONVGEMM_SUCCESS) return st;
         if (repack && !ws->ws_w) { ws->ws_w = base; ws->w_bytes = wt_bytes; }
         if (pbytes && !ws->ws_parts) { ws->ws_parts = (char *)base + wt_al; ws->parts_bytes = pbytes; }
@@ -858,8 +881,10 @@ static void dropin(const convgemm_desc_t *d, float beta, const float *in, const 
                    const convgemm_epilogue_t *epi, float *ac_pack, float *bc_pack) {
     cg_wsets ws = {0};
     /* Ac_pack holds split-K partials, Bc_pack the repacked weights */
-    if (ac_pack) { ws.ws_parts = ac_pack; ws.parts_bytes = pack_bytes_of(ac_pack); }
-    if (bc_pack) { ws.ws_w = bc_pack; ws.w_bytes = pack_bytes_of(bc_pack); }
+    ws.parts_bytes = pack_bytes_of(ac_pack);
+    ws.w_bytes = pack_bytes_of(bc_pack);
+    if (ws.parts_bytes) ws.ws_parts = ac_pack;
+    if (ws.w_bytes) ws.ws_w = bc_pack;
     hipStream_t s = g_stream;
     convgemm_status_t st = run(d, 1.0f, beta, in, x, out, epi, CONVGEMM_F32, NULL, &ws, s);
     if (st != CONVGEMM_SUCCESS) return;

@@ -1,8 +1,18 @@
 /*
  * costmodel.c -- learned cost model for tile-configuration selection.
  *
- * An MLP  F=26 features -> 64 -> 64 -> 1  (ReLU) predicts the standardised
- * log time of one (problem, candidate) pair.  It is trained on every timing
+ * An MLP  F=30 features -> 64 -> 64 -> 1  (ReLU) predicts the standardised
+ * log time of one (problem, candidate) pair.  Features (cg_features), log2
+ * where a size:
+ *   0-5   operation one-hot (forward / weight gradient / backward data x NHWC, NCHW)
+ *   6-8   operand type one-hot (FP32, BF16, FP8);  9  16-byte loads (vec)
+ *   10-14 pixels P, channels C, kn, taps, stride product
+ *   15-17 GEMM rows M, columns N, reduction length (K tiles x BK)
+ *   18-21 tile BM, BN, bytes per K step of a row, split-K factor
+ *   22    workgroups launched;  23  tile efficiency (useful / computed outputs)
+ *   24    waves of workgroups over 256 CUs at the tile's LDS occupancy
+ *   25    computed FLOPs (with tile padding)
+ *   26    pipelined kernel;  27  stream-K;  28  waves per workgroup;  29  LDS stages  It is trained on every timing
  * the autotuner recorded (tune.c) by the device kernels of mlp.hip --
  * forward, backward and Adam on the GPU -- and evaluated by the same forward
  * kernels to rank the candidates of a problem that was never tuned.  The

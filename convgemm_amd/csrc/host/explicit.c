@@ -9,7 +9,6 @@
  * W = op(A)^T.  An operand whose storage is not that packed row-major
  * matrix is copied (strided) or transposed into an internal workspace.
  */
-#include <pthread.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <string.h>
@@ -23,35 +22,11 @@
 
 #define AL256(x) (((size_t)(x) + 255) & ~(size_t)255)
 
-static pthread_mutex_t e_lock = PTHREAD_MUTEX_INITIALIZER;
-static void *e_ws = NULL;
-static size_t e_ws_bytes = 0;
-
-/* Grow-only packing workspace.  Growing waits for the device: queued work
- * may still read the old buffer.  Concurrent GEMM calls that pack share it
- * and must be ordered by the caller (one stream), like the reference's
- * shared pack buffers. */
-static convgemm_status_t pack_ws(size_t need, void **out) {
-    pthread_mutex_lock(&e_lock);
-    if (need > e_ws_bytes) {
-        if (e_ws) {
-            hipDeviceSynchronize();
-            hipFree(e_ws);
-        }
-        e_ws = NULL;
-        e_ws_bytes = 0;
-        size_t sz = need + (need >> 2);
-        if (cg_malloc(&e_ws, sz)) {
-            e_ws = NULL;
-            pthread_mutex_unlock(&e_lock);
-            return cg_set_error(CONVGEMM_ERR_ALLOC, "cannot allocate the GEMM packing workspace");
-        }
-        e_ws_bytes = sz;
-    }
-    *out = e_ws;
-    pthread_mutex_unlock(&e_lock);
-    return CONVGEMM_SUCCESS;
-}
+/* Grow-only packing workspace of the GEMM entry (cg_growbuf_get: outgrown
+ * buffers are retired, not freed, and it does not grow under capture).
+ * Concurrent GEMM calls that pack share it and must be ordered by the caller
+ * (one stream), like the reference's shared pack buffers. */
+static cg_growbuf e_ws = {NULL, 0, 0, "GEMM packing workspace"};
 
 /* ------------------------------------------------------------- transforms */
 
@@ -88,7 +63,7 @@ void im2row_nhwc(float *rows, int ld, const float *in, int batches, int height, 
         cg_set_error(CONVGEMM_ERR_INVALID_VALUE, "im2row_nhwc: ld < c*kh*kw");
         return;
     }
-    finish(cg_launch_im2row(&s, in, rows, ld, 0, cg_get_stream()), "im2row_nhwc");
+    finish(cg_launch_im2row(&s, in, rows, ld, 0, 4, cg_get_stream()), "im2row_nhwc");
 }
 
 void row2im_nhwc(int m, int n, const float *rows, int ld, float *out, int batches, int height, int width,
@@ -118,7 +93,7 @@ void im2col_nchw(float *cols, int ld, const float *in, int batches, int channels
         cg_set_error(CONVGEMM_ERR_INVALID_VALUE, "im2col_nchw: ld < b*ho*wo");
         return;
     }
-    finish(cg_launch_im2row(&s, in, cols, ld, 1, cg_get_stream()), "im2col_nchw");
+    finish(cg_launch_im2row(&s, in, cols, ld, 1, 4, cg_get_stream()), "im2col_nchw");
 }
 
 void col2im_nchw(int m, int n, const float *cols, int ld, float *out, int batches, int channels, int height,
@@ -146,6 +121,36 @@ void transpose_nchw(int rows, int cols, const float *in, int ld, float beta, flo
     finish(cg_launch_transpose_nchw(rows, cols, in, ld, beta, out, kn, ho, wo, start_row, start_col,
                                     cg_get_stream()),
            "transpose_nchw");
+}
+
+convgemm_status_t convgemm_transpose_nchw_ex(int rows, int cols, const float *in, long long ld, float beta,
+                                            float *out, int kn, int ho, int wo, int start_row, int start_col,
+                                            convgemm_stream_t stream) {
+    if (rows < 0 || cols < 0 || ho < 1 || wo < 1 || start_row < 0 || start_col < 0 || (cols > 0 && ld < rows))
+        return cg_set_error(CONVGEMM_ERR_INVALID_VALUE, "transpose_nchw_ex: invalid arguments");
+    int e = cg_launch_transpose_nchw(rows, cols, in, ld, beta, out, kn, ho, wo, start_row, start_col, stream);
+    return e ? cg_hip_fail(e, "transpose_nchw_ex") : cg_set_error(CONVGEMM_SUCCESS, "");
+}
+
+convgemm_status_t convgemm_im2row_ex(const convgemm_desc_t *d, convgemm_dtype_t dtype, const void *x, void *out,
+                                     long long ld, convgemm_stream_t stream) {
+    if (!d || (d->layout != CONVGEMM_NHWC && d->layout != CONVGEMM_NCHW))
+        return cg_set_error(CONVGEMM_ERR_INVALID_VALUE, "im2row_ex: NULL descriptor or unknown layout");
+    if (dtype != CONVGEMM_F32 && dtype != CONVGEMM_BF16)
+        return cg_set_error(CONVGEMM_ERR_INVALID_VALUE, "im2row_ex: FP32 or BF16 elements only");
+    const int ho = convgemm_out_size(d->h, d->vpad, d->vdil, d->kh, d->vstride);
+    const int wo = convgemm_out_size(d->w, d->hpad, d->hdil, d->kw, d->hstride);
+    cg_tshape s;
+    if (!tshape(&s, d->b, d->h, d->w, d->c, ho, wo, d->kh, d->kw, d->vpad, d->hpad, d->vstride, d->hstride, d->vdil,
+                d->hdil))
+        return convgemm_get_last_error();
+    const int nchw = d->layout == CONVGEMM_NCHW;
+    if (ld < (nchw ? (long long)d->b * ho * wo : (long long)d->c * d->kh * d->kw))
+        return cg_set_error(CONVGEMM_ERR_INVALID_VALUE, "im2row_ex: ld smaller than a patch-matrix row");
+    if ((long long)d->b * ho * wo * d->c * d->kh * d->kw > 0 && (!x || !out))
+        return cg_set_error(CONVGEMM_ERR_INVALID_VALUE, "im2row_ex: NULL tensor");
+    int e = cg_launch_im2row(&s, x, out, ld, nchw, dtype == CONVGEMM_BF16 ? 2 : 4, stream);
+    return e ? cg_hip_fail(e, "im2row_ex") : cg_set_error(CONVGEMM_SUCCESS, "");
 }
 
 /* ------------------------------------------------------------------- GEMM */
@@ -213,7 +218,7 @@ convgemm_status_t convgemm_gemm(char orderA, char orderB, char orderC, char tran
     size_t ny = c_packed ? 0 : AL256((size_t)P * Nn * 4);
     void *ws = NULL;
     if (nx + nw + ny) {
-        convgemm_status_t st = pack_ws(nx + nw + ny, &ws);
+        convgemm_status_t st = cg_growbuf_get(&e_ws, nx + nw + ny, s, &ws);
         if (st != CONVGEMM_SUCCESS) return st;
     }
     char *wp = (char *)ws;

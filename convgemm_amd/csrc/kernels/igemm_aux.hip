@@ -228,46 +228,6 @@ static void launch_reduce(const cg_kparams *kp, dim3 grid, hipStream_t st) {
 
 using namespace cg;
 
-// ---- per-type configuration tables (igemm_{f32,bf16,fp8}.hip) ----
-extern "C" {
-int cg_f32_num_cfgs(void);
-int cg_f32_get_cfg(int cfg, cg_tilecfg *out);
-int cg_f32_supports(int cfg, int op, int vec);
-int cg_f32_launch(int cfg, const cg_kparams *kp, void *stream);
-int cg_bf16_num_cfgs(void);
-int cg_bf16_get_cfg(int cfg, cg_tilecfg *out);
-int cg_bf16_supports(int cfg, int op, int vec);
-int cg_bf16_launch(int cfg, const cg_kparams *kp, void *stream);
-int cg_fp8_num_cfgs(void);
-int cg_fp8_get_cfg(int cfg, cg_tilecfg *out);
-int cg_fp8_supports(int cfg, int op, int vec);
-int cg_fp8_launch(int cfg, const cg_kparams *kp, void *stream);
-int cg_f32_launch_grouped(int cfg, const cg_kparams *kp0, const cg_group *g, void *stream);
-int cg_bf16_launch_grouped(int cfg, const cg_kparams *kp0, const cg_group *g, void *stream);
-int cg_fp8_launch_grouped(int cfg, const cg_kparams *kp0, const cg_group *g, void *stream);
-int cg_f32_occupancy(int cfg, const cg_kparams *kp0);
-int cg_bf16_occupancy(int cfg, const cg_kparams *kp0);
-int cg_fp8_occupancy(int cfg, const cg_kparams *kp0);
-}
-
-extern "C" int cg_launch_igemm_grouped(int elem, int cfg, const cg_kparams *kp0, const cg_group *g, void *stream) {
-    switch (elem) {
-    case CG_ELEM_F32: return cg_f32_launch_grouped(cfg, kp0, g, stream);
-    case CG_ELEM_BF16: return cg_bf16_launch_grouped(cfg, kp0, g, stream);
-    case CG_ELEM_FP8: return cg_fp8_launch_grouped(cfg, kp0, g, stream);
-    }
-    return (int)hipErrorInvalidValue;
-}
-
-extern "C" int cg_igemm_occupancy(int elem, int cfg, const cg_kparams *kp0) {
-    switch (elem) {
-    case CG_ELEM_F32: return cg_f32_occupancy(cfg, kp0);
-    case CG_ELEM_BF16: return cg_bf16_occupancy(cfg, kp0);
-    case CG_ELEM_FP8: return cg_fp8_occupancy(cfg, kp0);
-    }
-    return 0;
-}
-
 extern "C" int cg_igemm_sk_occupancy(int elem, int cfg, const cg_kparams *kp0) {
     cg_tilecfg t;
     if (cg_get_tilecfg(elem, cfg, &t)) return 0;
@@ -322,41 +282,9 @@ extern "C" int cg_launch_splitk_reduce_grouped(int op, const cg_group *g, void *
     return (int)hipGetLastError();
 }
 
-extern "C" int cg_num_tilecfgs(int elem) {
-    switch (elem) {
-    case CG_ELEM_F32: return cg_f32_num_cfgs();
-    case CG_ELEM_BF16: return cg_bf16_num_cfgs();
-    case CG_ELEM_FP8: return cg_fp8_num_cfgs();
-    }
-    return 0;
-}
 
-extern "C" int cg_get_tilecfg(int elem, int cfg, cg_tilecfg *out) {
-    switch (elem) {
-    case CG_ELEM_F32: return cg_f32_get_cfg(cfg, out);
-    case CG_ELEM_BF16: return cg_bf16_get_cfg(cfg, out);
-    case CG_ELEM_FP8: return cg_fp8_get_cfg(cfg, out);
-    }
-    return -1;
-}
 
-extern "C" int cg_tilecfg_supports(int elem, int cfg, int op, int vec) {
-    switch (elem) {
-    case CG_ELEM_F32: return cg_f32_supports(cfg, op, vec);
-    case CG_ELEM_BF16: return cg_bf16_supports(cfg, op, vec);
-    case CG_ELEM_FP8: return cg_fp8_supports(cfg, op, vec);
-    }
-    return 0;
-}
 
-extern "C" int cg_launch_igemm(int elem, int cfg, const cg_kparams *kp, void *stream) {
-    switch (elem) {
-    case CG_ELEM_F32: return cg_f32_launch(cfg, kp, stream);
-    case CG_ELEM_BF16: return cg_bf16_launch(cfg, kp, stream);
-    case CG_ELEM_FP8: return cg_fp8_launch(cfg, kp, stream);
-    }
-    return (int)hipErrorInvalidValue;
-}
 
 extern "C" int cg_launch_splitk_reduce(const cg_kparams *kp, void *stream) {
     hipStream_t st = (hipStream_t)stream;

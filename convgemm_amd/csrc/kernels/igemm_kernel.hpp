@@ -852,36 +852,4 @@ inline int launch_op(const cg_kparams *kp, const cg_group *g, int *occ, hipStrea
     }
 }
 
-template <typename T, int BM, int BN, int BK, int WM, int WN, bool NV>
-inline int launch_cfg(const cg_kparams *kp, const cg_group *g, int *occ, hipStream_t st) {
-    switch (kp->op) {
-    case CG_FWD_NHWC: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_FWD_NHWC>(kp, g, occ, st);
-    case CG_FWD_NCHW: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_FWD_NCHW>(kp, g, occ, st);
-    case CG_WGRAD_NHWC: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_WGRAD_NHWC>(kp, g, occ, st);
-    case CG_WGRAD_NCHW: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_WGRAD_NCHW>(kp, g, occ, st);
-    case CG_DGRAD_NHWC: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_DGRAD_NHWC>(kp, g, occ, st);
-    case CG_DGRAD_NCHW: return launch_op<T, BM, BN, BK, WM, WN, NV, CG_DGRAD_NCHW>(kp, g, occ, st);
-    }
-    return (int)hipErrorInvalidValue;
-}
-
-// lp != nullptr marks the configurations served by the pipelined forward
-// kernel (lp_kernel.hpp): forward NHWC with 16-byte channel chunks only.
-template <typename T>
-inline int supports(const cg_tilecfg *table, const bool *novec, int n, int cfg, int op, int vec,
-                    const bool *lp = nullptr, const bool *lpn = nullptr) {
-    if (cfg < 0 || cfg >= n || op < 0 || op >= CG_NUM_OPKINDS) return 0;
-    if (lp && lp[cfg]) return op == CG_FWD_NHWC && vec;
-    if (lpn && lpn[cfg]) return op == CG_FWD_NCHW && vec;
-    const bool en[6] = {op_enabled<T>(0), op_enabled<T>(1), op_enabled<T>(2),
-                        op_enabled<T>(3), op_enabled<T>(4), op_enabled<T>(5)};
-    const bool vr[6] = {vec_relevant<T>(0), vec_relevant<T>(1), vec_relevant<T>(2),
-                        vec_relevant<T>(3), vec_relevant<T>(4), vec_relevant<T>(5)};
-    if (!en[op]) return 0;
-    if (!vec && vr[op] && !novec[cfg]) return 0;
-    // tiles of 256x128 and up spill registers in the backward-data gather
-    if ((op == CG_DGRAD_NHWC || op == CG_DGRAD_NCHW) && table[cfg].BM * table[cfg].BN >= 256 * 128) return 0;
-    return 1;
-}
-
 }  // namespace cg

@@ -1,0 +1,104 @@
+// tiles.hip -- the tile-configuration tables (tile_tables.hpp) behind
+// cg_internal.h's configuration queries, and the dispatch of a launch to the
+// instantiation slice (slice.hip) holding its kernels.
+//
+// Slices are referenced weakly: the build decides which (type, operation,
+// family) slices exist, and a slice that was not built resolves to null here
+// and reports hipErrorInvalidValue (FP8 has forward slices only; the
+// pipelined family exists for the forward operations).
+#include "igemm_kernel.hpp"
+#include "tile_tables.hpp"
+
+using namespace cg;
+
+#define CG_SLICE_DECL(tn, op, fam) \
+    extern "C" __attribute__((weak)) int cg_slice_##tn##_##op##_##fam(int, const cg_kparams *, const cg_group *, int *, void *);
+#define CG_SLICE_DECL_T(tn)                                                                                        \
+    CG_SLICE_DECL(tn, 0, 0) CG_SLICE_DECL(tn, 1, 0) CG_SLICE_DECL(tn, 2, 0) CG_SLICE_DECL(tn, 3, 0)               \
+    CG_SLICE_DECL(tn, 4, 0) CG_SLICE_DECL(tn, 5, 0) CG_SLICE_DECL(tn, 0, 1) CG_SLICE_DECL(tn, 1, 1)
+CG_SLICE_DECL_T(f32)
+CG_SLICE_DECL_T(bf16)
+CG_SLICE_DECL_T(fp8)
+
+namespace {
+
+#define CG_SLICE_ROW(tn)                                                                                           \
+    {{cg_slice_##tn##_0_0, cg_slice_##tn##_0_1}, {cg_slice_##tn##_1_0, cg_slice_##tn##_1_1},                       \
+     {cg_slice_##tn##_2_0, nullptr}, {cg_slice_##tn##_3_0, nullptr},                                               \
+     {cg_slice_##tn##_4_0, nullptr}, {cg_slice_##tn##_5_0, nullptr}}
+// [elem][op][family: 0 general, 1 pipelined]
+const slice_fn kSlices[3][CG_NUM_OPKINDS][2] = {CG_SLICE_ROW(f32), CG_SLICE_ROW(bf16), CG_SLICE_ROW(fp8)};
+
+template <typename T> const TileDef *table(int cfg) {
+    return cfg >= 0 && cfg < num_tiles<T>() ? &Tiles<T>::defs[cfg] : nullptr;
+}
+
+const TileDef *def_of(int elem, int cfg) {
+    switch (elem) {
+    case CG_ELEM_F32: return table<float>(cfg);
+    case CG_ELEM_BF16: return table<uint16_t>(cfg);
+    case CG_ELEM_FP8: return table<uint8_t>(cfg);
+    }
+    return nullptr;
+}
+
+template <typename T> int supports_t(const TileDef &d, int op, int vec) {
+    if (d.fam == FAM_LP) return op == CG_FWD_NHWC && vec;
+    if (d.fam == FAM_LPN) return op == CG_FWD_NCHW && vec;
+    if (!op_enabled<T>(op)) return 0;
+    if (!vec && vec_relevant<T>(op) && !d.nv) return 0;
+    // tiles of 256x128 and up spill registers in the backward-data gather
+    if ((op == CG_DGRAD_NHWC || op == CG_DGRAD_NCHW) && d.BM * d.BN >= 256 * 128) return 0;
+    return 1;
+}
+
+int launch_any(int elem, int cfg, const cg_kparams *kp, const cg_group *g, int *occ, void *stream) {
+    const TileDef *d = def_of(elem, cfg);
+    if (!d || kp->op < 0 || kp->op >= CG_NUM_OPKINDS) return (int)hipErrorInvalidValue;
+    const slice_fn f = kSlices[elem][kp->op][d->fam == FAM_GEN ? 0 : 1];
+    if (!f) return (int)hipErrorInvalidValue;
+    return f(cfg, kp, g, occ, stream);
+}
+
+}  // namespace
+
+extern "C" int cg_num_tilecfgs(int elem) {
+    switch (elem) {
+    case CG_ELEM_F32: return num_tiles<float>();
+    case CG_ELEM_BF16: return num_tiles<uint16_t>();
+    case CG_ELEM_FP8: return num_tiles<uint8_t>();
+    }
+    return 0;
+}
+
+extern "C" int cg_get_tilecfg(int elem, int cfg, cg_tilecfg *out) {
+    const TileDef *d = def_of(elem, cfg);
+    if (!d) return -1;
+    *out = cg_tilecfg{d->BM, d->BN, d->BK, d->WM, d->WN, d->NS};
+    return 0;
+}
+
+extern "C" int cg_tilecfg_supports(int elem, int cfg, int op, int vec) {
+    const TileDef *d = def_of(elem, cfg);
+    if (!d || op < 0 || op >= CG_NUM_OPKINDS) return 0;
+    switch (elem) {
+    case CG_ELEM_F32: return supports_t<float>(*d, op, vec);
+    case CG_ELEM_BF16: return supports_t<uint16_t>(*d, op, vec);
+    case CG_ELEM_FP8: return supports_t<uint8_t>(*d, op, vec);
+    }
+    return 0;
+}
+
+extern "C" int cg_launch_igemm(int elem, int cfg, const cg_kparams *kp, void *stream) {
+    return launch_any(elem, cfg, kp, nullptr, nullptr, stream);
+}
+
+extern "C" int cg_launch_igemm_grouped(int elem, int cfg, const cg_kparams *kp0, const cg_group *g, void *stream) {
+    return launch_any(elem, cfg, kp0, g, nullptr, stream);
+}
+
+extern "C" int cg_igemm_occupancy(int elem, int cfg, const cg_kparams *kp0) {
+    int occ = 0;
+    if (launch_any(elem, cfg, kp0, nullptr, &occ, nullptr) != 0) return 0;
+    return occ;
+}

@@ -3,9 +3,9 @@
 // The reference materialises its patch matrices only in its tests and in the
 // explicit path (src/im2row_nhwc.c:147-229, src/im2col_nchw.c:146-233,
 // :206-233, :337-359); here they are bandwidth-bound grid-stride kernels:
-//   im2row / im2col: one thread per patch-matrix element, the element index
-//     fastest along the matrix row so the stores coalesce; only in-bounds
-//     taps are written (the caller zero-fills, as the reference's tests do);
+//   im2row / im2col: 4- or 2-byte elements, threads along the patch-matrix
+//     row so the stores coalesce; only in-bounds taps are written (the caller
+//     zero-fills, as the reference's tests do);
 //   row2im / col2im: a gather per image element over the taps that reach it
 //     (no atomics, fixed order), added to the image;
 //   transpose_nchw: [kn-block][b][ho][wo] -> [b][kn][ho][wo] with beta;
@@ -24,27 +24,49 @@ inline unsigned grid_for(long long n) {
     return (unsigned)(g > 0 ? g : 1);
 }
 
-// rows (NHWC: P x K, ld) or cols (NCHW: K x P, ld) from the image x.
-__global__ void im2row_kernel(cg_tshape s, const float *__restrict__ x, float *__restrict__ out, long long ld,
-                              int nchw) {
-    const long long taps = (long long)s.kh * s.kw;
-    const long long K = (long long)s.c * taps;
-    const long long P = (long long)s.b * s.ho * s.wo;
-    const long long total = P * K;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (long long)gridDim.x * blockDim.x) {
-        long long p, r;
-        if (nchw) { r = i / P; p = i - r * P; }
-        else { p = i / K; r = i - p * K; }
-        const int c = (int)(r / taps), t = (int)(r - (long long)c * taps);
+// rows (NHWC: P x K, ld) from the image x: a block walks patch rows p
+// (grid-stride), its threads the K = c*kh*kw elements of a row (r = (c*kh +
+// kx)*kw + ky), so the stores of a row coalesce; the pixel decomposition is
+// done once per row and the per-element index math is 32-bit.  E is the
+// element type (4 or 2 bytes: the copy does no arithmetic).
+template <typename E>
+__global__ void im2row_nhwc_kernel(cg_tshape s, const E *__restrict__ x, E *__restrict__ out, long long ld) {
+    const int taps = s.kh * s.kw, K = s.c * taps, hw = s.ho * s.wo;
+    const long long P = (long long)s.b * hw;
+    for (long long p = blockIdx.x; p < P; p += gridDim.x) {
+        const int b = (int)(p / hw), rem = (int)(p - (long long)b * hw);
+        const int xo = rem / s.wo, yo = rem - xo * s.wo;
+        const int ix0 = s.vs * xo - s.vp, iy0 = s.hs * yo - s.hp;
+        const E *img = x + (long long)b * s.h * s.w * s.c;
+        E *row = out + p * ld;
+        for (int r = threadIdx.x; r < K; r += blockDim.x) {
+            const int c = r / taps, t = r - c * taps;
+            const int kx = t / s.kw, ky = t - kx * s.kw;
+            const int ix = ix0 + s.vd * kx, iy = iy0 + s.hd * ky;
+            if ((unsigned)ix < (unsigned)s.h && (unsigned)iy < (unsigned)s.w) row[r] = img[((long long)ix * s.w + iy) * s.c + c];
+        }
+    }
+}
+
+// cols (NCHW: K x P, ld) from the image x: a block walks the pixels of one
+// patch row r (grid-stride over (r, pixel block)), threads along p, so the
+// stores coalesce and the loads run along an image row.
+template <typename E>
+__global__ void im2col_nchw_kernel(cg_tshape s, const E *__restrict__ x, E *__restrict__ out, long long ld) {
+    const int taps = s.kh * s.kw, K = s.c * taps, hw = s.ho * s.wo;
+    const long long P = (long long)s.b * hw;
+    const long long pblocks = (P + blockDim.x - 1) / blockDim.x;
+    for (long long w = blockIdx.x; w < (long long)K * pblocks; w += gridDim.x) {
+        const int r = (int)(w / pblocks);
+        const long long p = (w - (long long)r * pblocks) * blockDim.x + threadIdx.x;
+        if (p >= P) continue;
+        const int c = r / taps, t = r - c * taps;
         const int kx = t / s.kw, ky = t - kx * s.kw;
-        const int hw = s.ho * s.wo;
         const int b = (int)(p / hw), rem = (int)(p - (long long)b * hw);
         const int xo = rem / s.wo, yo = rem - xo * s.wo;
         const int ix = s.vs * xo + s.vd * kx - s.vp, iy = s.hs * yo + s.hd * ky - s.hp;
-        if (ix < 0 || ix >= s.h || iy < 0 || iy >= s.w) continue;
-        if (nchw) out[r * ld + p] = x[(((long long)b * s.c + c) * s.h + ix) * s.w + iy];
-        else out[p * ld + r] = x[(((long long)b * s.h + ix) * s.w + iy) * s.c + c];
+        if ((unsigned)ix < (unsigned)s.h && (unsigned)iy < (unsigned)s.w)
+            out[(long long)r * ld + p] = x[(((long long)b * s.c + c) * s.h + ix) * s.w + iy];
     }
 }
 
@@ -121,11 +143,29 @@ __global__ void copy2d_kernel(const E *__restrict__ src, long long lds, E *__res
 
 }  // namespace
 
-extern "C" int cg_launch_im2row(const cg_tshape *s, const float *x, float *out, long long ld, int nchw,
+extern "C" int cg_launch_im2row(const cg_tshape *s, const void *x, void *out, long long ld, int nchw, int esize,
                                 void *stream) {
-    const long long n = (long long)s->b * s->ho * s->wo * s->c * s->kh * s->kw;
-    if (n <= 0) return 0;
-    hipLaunchKernelGGL(im2row_kernel, dim3(grid_for(n)), dim3(TB), 0, (hipStream_t)stream, *s, x, out, ld, nchw);
+    const long long P = (long long)s->b * s->ho * s->wo, K = (long long)s->c * s->kh * s->kw;
+    if (P <= 0 || K <= 0) return 0;
+    if (esize != 4 && esize != 2) return (int)hipErrorInvalidValue;
+    hipStream_t st = (hipStream_t)stream;
+    if (!nchw) {
+        const unsigned g = (unsigned)(P < 65536 ? P : 65536);
+        if (esize == 4)
+            hipLaunchKernelGGL(im2row_nhwc_kernel<uint32_t>, dim3(g), dim3(TB), 0, st, *s, (const uint32_t *)x,
+                               (uint32_t *)out, ld);
+        else
+            hipLaunchKernelGGL(im2row_nhwc_kernel<uint16_t>, dim3(g), dim3(TB), 0, st, *s, (const uint16_t *)x,
+                               (uint16_t *)out, ld);
+    } else {
+        const unsigned g = grid_for(P * K);
+        if (esize == 4)
+            hipLaunchKernelGGL(im2col_nchw_kernel<uint32_t>, dim3(g), dim3(TB), 0, st, *s, (const uint32_t *)x,
+                               (uint32_t *)out, ld);
+        else
+            hipLaunchKernelGGL(im2col_nchw_kernel<uint16_t>, dim3(g), dim3(TB), 0, st, *s, (const uint16_t *)x,
+                               (uint16_t *)out, ld);
+    }
     return (int)hipGetLastError();
 }
 

@@ -35,7 +35,10 @@ extern "C" {
 /* Replaces alloc_pack_buffs (src/convGemm.h:20, src/convGemm.c:30-40).
  * Allocates the two device workspaces used by the conv calls (split-K
  * partial sums and the repacked weights).  Returns 0 on success, 1 if an
- * allocation failed.  Release them with convgemm_free_pack_buffs(). */
+ * allocation failed.  Release them with convgemm_free_pack_buffs().
+ * The conv calls use ac_pack / bc_pack only when they came from this
+ * function (the library knows their size); any other pointer, like NULL,
+ * makes the call use the library's internal workspace. */
 int alloc_pack_buffs(float **Ac_pack, float **Bc_pack);
 
 /* Replaces sconvGemmNHWC (src/convGemm.h:22-33, src/convGemm.c:48-77).
@@ -137,7 +140,13 @@ typedef void *convgemm_stream_t;
  * (the drop-in calls are alpha=1, beta=0 for FWD/WGRAD and beta=1 for DGRAD).
  * alpha == 0 follows quick_gemm.h: out = beta*out (0 if beta == 0) and the
  * epilogue is not applied.  `epi` may be NULL.  `workspace`/`ws_bytes` may
- * be NULL/0, in which case the library uses an internal device buffer.
+ * be NULL/0, in which case the library uses an internal device buffer: one
+ * grow-only buffer shared by every call that passes no workspace, so such
+ * calls on different streams must not overlap in time.  An outgrown internal
+ * buffer is kept allocated (graphs captured earlier may still use it), and
+ * while `stream` is being captured the buffer does not grow: the call then
+ * returns CONVGEMM_ERR_WORKSPACE, and calls made under capture pass a
+ * workspace of convgemm_workspace_size() bytes.
  * `scales` (FP8 only): {scale_in, scale_x}, the true values are
  * stored*scale. */
 convgemm_status_t convgemm_conv_ex(const convgemm_desc_t *desc, float alpha, float beta,

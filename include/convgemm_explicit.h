@@ -55,6 +55,22 @@ void col2im_nchw(int m, int n, const float *cols, int ld, float *out, int batche
 void transpose_nchw(int rows, int cols, const float *in, int ld, float beta, float *out, int kn, int ho, int wo,
                     int start_row, int start_col);
 
+/* Extended form of im2row_nhwc / im2col_nchw (no counterpart in the
+ * reference, whose transforms are FP32 only): the patch matrix of a
+ * descriptor's input with FP32 or BF16 elements (dtype; the copy does no
+ * arithmetic), asynchronous on `stream`.  NHWC: rows[p*ld + r] with ld >=
+ * c*kh*kw; NCHW: cols[r*ld + p] with ld >= b*ho*wo; p and r as above.  Only
+ * in-bounds taps are written (the caller zero-fills).  The weight gradient's
+ * and backward data's descriptors describe the same input. */
+convgemm_status_t convgemm_im2row_ex(const convgemm_desc_t *desc, convgemm_dtype_t dtype, const void *x, void *out,
+                                     long long ld, convgemm_stream_t stream);
+
+/* Extended form of transpose_nchw: the same operation, asynchronous on
+ * `stream` (the drop-in form waits for the library stream). */
+convgemm_status_t convgemm_transpose_nchw_ex(int rows, int cols, const float *in, long long ld, float beta,
+                                            float *out, int kn, int ho, int wo, int start_row, int start_col,
+                                            convgemm_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Plain GEMM and layout converters (asynchronous on `stream`)                */
 /* ------------------------------------------------------------------------ */

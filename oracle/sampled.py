@@ -60,8 +60,9 @@ def wgrad_sampled(layout, s: Shape, dY, X, samples):
         iy = ys * s.hstr + ky * s.hdil - s.hpad
         mx = (ix >= 0) & (ix < s.h)
         my = (iy >= 0) & (iy < s.w)
-        xv = Xn[:, ix[mx]][:, :, iy[my], c].astype(f64)  # (b, nx, ny)
-        yv = Yn[:, xs[mx]][:, :, ys[my], k].astype(f64)
+        # (b, nx, ny) planes of channel c / k only (no copy of the other channels)
+        xv = Xn[:, ix[mx][:, None], iy[my][None, :], c].astype(f64)
+        yv = Yn[:, xs[mx][:, None], ys[my][None, :], k].astype(f64)
         out[i] = np.sum(xv * yv)
     return out
 

@@ -135,25 +135,27 @@ SHIPPED_KINDS = ["f32:fwd:nchw", "f32:dgrad:nhwc", "bf16:wgrad:nhwc", "fp8:fwd:n
 
 
 def test_shipped_model_beats_analytic():
-    """The model shipped with the library (the start-up default) on 96 fresh shapes
-    from its training ranges (24 of each of four kinds, from a seed it was not
-    trained on): every plan it picks is among the timed candidates, its geomean
-    regret against exhaustive tuning over all of them is below the analytic
-    planner's, and on no kind is it more than 5% above the analytic planner's."""
+    """The model shipped with the library (the start-up default) on 256 fresh shapes
+    from its training ranges (64 of each of four kinds, from a seed it was not
+    trained on): every plan it picks is among the timed candidates, and on EVERY
+    kind its geomean regret against exhaustive tuning is below the analytic
+    ("no learning") planner's."""
     probs = {}
     for mix in SHIPPED_KINDS:
         (_, dt, op, lay, _), = tuning.parse_mix(mix, 0)
-        probs[mix] = tuning.random_problems(24, seed=4242, dtype=dt, op=op, layout=lay)
+        probs[mix] = tuning.random_problems(64, seed=4242, dtype=dt, op=op, layout=lay)
     pooled = [p for v in probs.values() for p in v]
-    tuning.tune_all(pooled, reps=3, log_every=0)
+    tuning.tune_all(pooled, reps=3, log_every=32)
     cg.costmodel_load(SHIPPED_MODEL)
     r_m = tuning.regret(pooled, None, cg.PLAN_MODEL)
     r_a = tuning.regret(pooled, None, cg.PLAN_ANALYTIC)
     print("pooled regret model", r_m, "analytic", r_a, flush=True)
     assert r_m["n"] == len(pooled) and r_m["outside"] == 0
-    assert r_m["geomean"] < r_a["geomean"], (r_m, r_a)
+    worse = []
     for mix, ps in probs.items():
         m = tuning.regret(ps, None, cg.PLAN_MODEL)
         a = tuning.regret(ps, None, cg.PLAN_ANALYTIC)
         print(mix, "regret model", m["geomean"], "analytic", a["geomean"], flush=True)
-        assert m["geomean"] <= 1.05 * a["geomean"], (mix, m, a)
+        if not m["geomean"] < a["geomean"]:
+            worse.append((mix, m["geomean"], a["geomean"]))
+    assert not worse, worse

@@ -62,6 +62,36 @@ def test_im2row_im2col_exact(si):
         assert np.all(got[~mask] == 7.0)  # padded taps untouched
 
 
+@pytest.mark.parametrize("dtype", [cg.F32, cg.BF16])
+@pytest.mark.parametrize("si", range(len(SHAPES)))
+def test_im2row_ex_exact(si, dtype):
+    """convgemm_im2row_ex: the same copy in FP32 or BF16 elements, bit-exact against the
+    oracle's transform of the same values, padded taps untouched, a padded leading dimension."""
+    s = SHAPES[si]
+    rng = np.random.default_rng(50 + si)
+    for layout in (NHWC, NCHW):
+        X = rng.standard_normal(s.x_shape(layout)).astype(np.float32)
+        if dtype == cg.BF16:  # values exactly representable in BF16
+            X = torch.from_numpy(X).to(torch.bfloat16).float().numpy()
+        tdt = torch.float32 if dtype == cg.F32 else torch.bfloat16
+        if layout == NHWC:
+            ref, mask = oracle.im2row_nhwc(s, X), oracle.im2row_nhwc(s, np.ones_like(X)) != 0
+            rows, cols = s.P, s.K
+        else:
+            ref, mask = oracle.im2col_nchw(s, X), oracle.im2col_nchw(s, np.ones_like(X)) != 0
+            rows, cols = s.K, s.P
+        ld = cols + 3
+        buf = torch.full((rows, ld), 7.0, device=DEV, dtype=tdt)
+        x = dev(X).to(tdt)
+        d = cg.make_desc(cg.FWD, layout, s.b, s.h, s.w, s.c, s.kn, s.kh, s.kw, s.vpad, s.hpad, s.vstr, s.hstr,
+                         s.vdil, s.hdil)
+        cg.im2row_ex(d, x.view(torch.int16) if dtype == cg.BF16 else x,
+                     buf.view(torch.int16) if dtype == cg.BF16 else buf, ld, dtype=dtype)
+        got = host(buf.float())
+        assert np.array_equal(got[:, :cols][mask], ref[mask].astype(np.float32)), (si, layout)
+        assert np.all(got[:, :cols][~mask] == 7.0) and np.all(got[:, cols:] == 7.0)
+
+
 @pytest.mark.parametrize("si", range(len(SHAPES)))
 def test_row2im_col2im(si):
     s = SHAPES[si]
