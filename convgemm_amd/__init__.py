@@ -43,7 +43,7 @@ EXPORTED_SYMBOLS = (
     "convgemm_conv_ex", "convgemm_workspace_size", "convgemm_out_size", "convgemm_free_pack_buffs",
     "convgemm_set_stream", "convgemm_get_last_error", "convgemm_last_error_message", "convgemm_status_string",
     "convgemm_version", "convgemm_quantize", "convgemm_packed_weights_size", "convgemm_pack_weights",
-    "convgemm_conv_packed",
+    "convgemm_conv_packed", "convgemm_conv_noalloc", "convgemm_noalloc_scratch_bytes",
     # convgemm_tune.h
     "convgemm_set_planner", "convgemm_get_planner", "convgemm_autotune", "convgemm_tune_count", "convgemm_tune_save",
     "convgemm_tune_load",
@@ -225,6 +225,11 @@ def lib() -> ctypes.CDLL:
     L.convgemm_transpose_nchw_ex.restype = ci
     L.cg_set_chunk_window.argtypes = [ctypes.c_longlong]
     L.cg_set_chunk_window.restype = None
+    L.convgemm_conv_noalloc.argtypes = [ctypes.POINTER(Desc), cf, cf, vp, vp, vp, ctypes.POINTER(Epilogue), ci, vp,
+                                        vp]
+    L.convgemm_conv_noalloc.restype = ci
+    L.convgemm_noalloc_scratch_bytes.argtypes = []
+    L.convgemm_noalloc_scratch_bytes.restype = ctypes.c_size_t
     _lib = L
     return L
 
@@ -339,6 +344,23 @@ def conv_ex(desc: Desc, in_, x, out, alpha=1.0, beta=0.0, bias=None, mean=None, 
                             ctypes.byref(epi), int(dtype), sc, _ptr(workspace), int(ws_bytes), sh)
     if st != 0:
         raise ConvGemmError(st, L.convgemm_last_error_message().decode())
+
+
+def conv_noalloc(desc: Desc, in_, x, out, alpha=1.0, beta=0.0, bias=None, mean=None, inv_std=None, gamma=None,
+                 bnbeta=None, relu=False, dtype=F32, scales=None, stream=None):
+    """convgemm_conv_noalloc: conv_ex on the library's static device scratch (no device allocation);
+    calls that overlap in time on one GPU share that scratch and race."""
+    L = lib()
+    epi = Epilogue(_ptr(bias), _ptr(mean), _ptr(inv_std), _ptr(gamma), _ptr(bnbeta), int(bool(relu)))
+    st = L.convgemm_conv_noalloc(ctypes.byref(desc), float(alpha), float(beta), _ptr(in_), _ptr(x), _ptr(out),
+                                 ctypes.byref(epi), int(dtype), _scales_ptr(scales), _stream_handle(stream))
+    if st != 0:
+        raise ConvGemmError(st, L.convgemm_last_error_message().decode())
+
+
+def noalloc_scratch_bytes() -> int:
+    """Size of the static device scratch behind conv_noalloc."""
+    return int(lib().convgemm_noalloc_scratch_bytes())
 
 
 def im2row_ex(desc: Desc, x, out, ld, dtype=F32, stream=None) -> None:

@@ -143,6 +143,12 @@ int cg_launch_scale(float *out, long long n, float beta, void *stream);
 int cg_launch_quantize(const float *src, void *dst, long long n, int elem, void *scratch, void *stream);
 int cg_launch_permute3(const void *src, void *dst, int d0, int d1, int d2, int perm, int esize, void *stream);
 
+/* ---- static device scratch of the allocation-free entry point ---- */
+#define CG_SCRATCH_BYTES (32 << 20)
+/* Device address and size of the current device's scratch array (looked up
+ * once per device with hipGetSymbolAddress).  Returns 0 or a hipError_t. */
+int cg_scratch_get(void **ptr, size_t *bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -423,6 +423,7 @@ typedef struct {
     void *ws_w;
     size_t w_bytes;
     int prepacked; /* `in` already holds the weights in the kernels' order */
+    int noalloc;   /* the allocation-free entry point: no internal workspace, no planner device work */
 } cg_wsets;
 
 /* Weight order the kernels read, and the repack that produces it
@@ -644,9 +645,9 @@ static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, 
 
     /* plan on the first chunk's shape */
     int cfg, splits;
-    /* under capture the planner may not run device work (the cost model's
-     * inference allocates and copies) */
-    t_in_capture = cg_stream_capturing(stream);
+    /* under capture, or in the allocation-free entry point, the planner may
+     * not run device work (the cost model's inference allocates and copies) */
+    t_in_capture = ws->noalloc || cg_stream_capturing(stream);
     choose_plan(elem, &kp, &cfg, &splits);
     t_in_capture = 0;
     cg_tilecfg t;
@@ -667,6 +668,8 @@ static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, 
         pbytes = cg_partial_bytes(&kp);
     }
     if ((repack && !ws->ws_w) || (pbytes && !ws->ws_parts)) {
+        if (ws->noalloc) /* not reached: the scratch provides both parts */
+            return set_error(CONVGEMM_ERR_WORKSPACE, "allocation-free call needs a workspace it does not have");
         size_t wt_al = (wt_bytes + 255) & ~(size_t)255;
         void *base = NULL;
         st = internal_ws(wt_al + pbytes, stream, &base);
@@ -785,6 +788,49 @@ convgemm_status_t convgemm_conv_ex(const convgemm_desc_t *desc, float alpha, flo
     }
     return run(desc, alpha, beta, in, x, out, epi, dtype, scales, &ws, (hipStream_t)stream);
 }
+
+/* convgemm_conv_ex on the static device scratch (scratch.hip) instead of any
+ * workspace: no device allocation, no free, no planner device work (plans
+ * come from a forced plan, the tuning database, the cost model's cache of
+ * earlier choices, else the analytic planner).  The repacked weights take the
+ * front of the scratch; split-K partials get the rest, and plans whose
+ * partials do not fit fall back to fewer splits (stream-K to plain tiles).
+ * Every call on a device shares that device's scratch: calls that overlap in
+ * time on one GPU (other streams, other threads) race.  Nothing in the
+ * scratch has to start at zero: every partial a reduction reads is written
+ * by the same call first. */
+convgemm_status_t convgemm_conv_noalloc(const convgemm_desc_t *desc, float alpha, float beta, const void *in,
+                                        const void *x, void *out, const convgemm_epilogue_t *epi,
+                                        convgemm_dtype_t dtype, const float *scales, convgemm_stream_t stream) {
+    int ho, wo;
+    convgemm_status_t st = validate(desc, &ho, &wo);
+    if (st != CONVGEMM_SUCCESS) return st;
+    if (dtype != CONVGEMM_F32 && dtype != CONVGEMM_BF16 && dtype != CONVGEMM_FP8_E4M3)
+        return set_error(CONVGEMM_ERR_INVALID_VALUE, "unknown dtype %d", (int)dtype);
+    void *scr = NULL;
+    size_t sbytes = 0;
+    int e = cg_scratch_get(&scr, &sbytes);
+    if (e) return hip_fail((hipError_t)e, "device address of the static scratch");
+    int d0, d1, d2, perm;
+    const int elem = elem_of(dtype);
+    const size_t wt = weight_repack(desc, elem, &d0, &d1, &d2, &perm)
+                          ? (size_t)desc->c * desc->kh * desc->kw * desc->kn * esize_of(elem) : 0;
+    const size_t wt_al = (wt + 255) & ~(size_t)255;
+    if (wt_al > sbytes)
+        return set_error(CONVGEMM_ERR_WORKSPACE,
+                         "allocation-free call: the repacked weights need %zu bytes, more than the %zu-byte static "
+                         "scratch; use convgemm_conv_ex with a workspace, or convgemm_conv_packed",
+                         wt_al, sbytes);
+    cg_wsets ws = {0};
+    ws.noalloc = 1;
+    ws.ws_w = wt ? scr : NULL;
+    ws.w_bytes = wt;
+    ws.ws_parts = (char *)scr + wt_al;
+    ws.parts_bytes = sbytes - wt_al;
+    return run(desc, alpha, beta, in, x, out, epi, dtype, scales, &ws, (hipStream_t)stream);
+}
+
+size_t convgemm_noalloc_scratch_bytes(void) { return (size_t)CG_SCRATCH_BYTES; }
 
 convgemm_status_t convgemm_plan(const convgemm_desc_t *d, convgemm_dtype_t dtype, int *cfg, int *splits) {
     cg_kparams kp;

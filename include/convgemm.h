@@ -155,6 +155,24 @@ convgemm_status_t convgemm_conv_ex(const convgemm_desc_t *desc, float alpha, flo
                                    const float *scales,
                                    void *workspace, size_t ws_bytes, convgemm_stream_t stream);
 
+/* No counterpart in the reference: convgemm_conv_ex for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops and
+ * harnesses that forbid hipMalloc).  It makes no device allocation and no
+ * free: its repacked weights and split-K partials live in a static device
+ * array of convgemm_noalloc_scratch_bytes() (32 MiB) per GPU, held until the
+ * process ends.  Weights whose repacked copy exceeds the scratch are rejected
+ * with CONVGEMM_ERR_WORKSPACE; plans whose partials do not fit use fewer
+ * split-K slices.  Plans come from a forced plan, the tuning database or the
+ * cost model's cached choices, else the analytic planner (no model inference,
+ * which allocates).  Every call on one GPU shares that GPU's scratch, so calls
+ * that overlap in time on one GPU (other streams, other threads) race. */
+convgemm_status_t convgemm_conv_noalloc(const convgemm_desc_t *desc, float alpha, float beta,
+                                        const void *in, const void *x, void *out,
+                                        const convgemm_epilogue_t *epi, convgemm_dtype_t dtype,
+                                        const float *scales, convgemm_stream_t stream);
+/* Bytes of the static device scratch behind convgemm_conv_noalloc. */
+size_t convgemm_noalloc_scratch_bytes(void);
+
 /* Weight prepacking for repeated calls with the same weights (no counterpart
  * in the reference, which packs on every call).  Some operations read the
  * weights in another order than the caller's (forward: [kn][kh][kw][c],
