@@ -63,6 +63,8 @@ typedef struct {
     float alpha, beta;
     float scale;     /* product of operand scales (FP8), 1 otherwise */
     int sk;          /* stream-K workgroups (FWD/WGRAD, splits == 1); 0: off */
+    int sk_dp;       /* with sk: tiles 0 .. sk_dp-1 run one per workgroup (blocks 0 .. sk_dp-1), the
+                        sk workgroups after them share the K steps of tiles sk_dp .. mtiles*ntiles-1 */
 } cg_kparams;
 
 /* Tile configurations of the implicit-GEMM kernels. */

@@ -52,8 +52,11 @@ int cg_esize(int elem);
 int cg_malloc(void **p, size_t bytes);
 convgemm_status_t cg_hip_fail(int err, const char *what);
 /* Stream-K: workgroups for a problem with tile fields filled (0: none for
- * this configuration/operation), and the bytes of their partial tiles. */
+ * this configuration/operation, or no tile left to cut), and the bytes of
+ * their partial tiles.  cg_sk_split also gives the data-parallel tiles that
+ * precede the stream-K region. */
 int cg_sk_groups(int elem, int cfg, const cg_kparams *kp);
+int cg_sk_split(int elem, int cfg, const cg_kparams *kp, int *dp);
 size_t cg_sk_bytes(int groups, const cg_tilecfg *t);
 /* The library stream of the drop-in calls (hipStream_t). */
 void *cg_get_stream(void);

@@ -567,15 +567,32 @@ static int num_cus(void) {
     return n;
 }
 
-/* Stream-K workgroups for a problem with tiles filled (0 when the
- * configuration or operation has no stream-K kernel). */
-int cg_sk_groups(int elem, int cfg, const cg_kparams *kp) {
-    int occ = cg_igemm_sk_occupancy(elem, cfg, kp);
-    if (occ <= 0) return 0;
-    long long g = (long long)occ * num_cus();
-    long long units = (long long)kp->mtiles * kp->ntiles * kp->KT;
-    if (g > units) g = units;
+/* The stream-K schedule of a problem with tiles filled: data-parallel tiles
+ * and stream-K workgroups.  Whole rounds of tiles, one per CU, run one tile
+ * per workgroup (*dp, a multiple of the CU count); only the remaining
+ * T mod CUs tiles are cut along K, evenly over at most one workgroup per CU
+ * with at least SK_MIN_STEPS K steps each, so every CU gets T*KT/CUs K steps
+ * instead of ceil(T/CUs)*KT, and only the remainder tiles need a fix-up.
+ * Returns the stream-K workgroups (0: nothing to cut, or no stream-K kernel
+ * for this configuration/operation). */
+#define SK_MIN_STEPS 4
+int cg_sk_split(int elem, int cfg, const cg_kparams *kp, int *dp) {
+    *dp = 0;
+    if (cg_igemm_sk_occupancy(elem, cfg, kp) <= 0) return 0;
+    const long long cus = num_cus();
+    const long long tiles = (long long)kp->mtiles * kp->ntiles;
+    const long long rem = tiles % cus;
+    *dp = (int)(tiles - rem);
+    if (rem == 0 || kp->KT < 1) return 0;
+    long long g = rem * kp->KT / SK_MIN_STEPS;
+    if (g > cus) g = cus;
+    if (g < 1) g = 1;
     return (int)g;
+}
+
+int cg_sk_groups(int elem, int cfg, const cg_kparams *kp) {
+    int dp;
+    return cg_sk_split(elem, cfg, kp, &dp);
 }
 
 size_t cg_sk_bytes(int groups, const cg_tilecfg *t) { return (size_t)groups * 2 * t->BM * t->BN * sizeof(float); }
@@ -654,8 +671,9 @@ static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, 
     if (cfg < 0 || cg_get_tilecfg(elem, cfg, &t))
         return set_error(CONVGEMM_ERR_UNSUPPORTED, "no kernel configuration for this problem");
     cg_fill_tiles(&kp, &t, splits > 0 ? splits : 1);
-    /* splits == 0: stream-K (forward and weight gradient, general kernel) */
-    kp.sk = splits == 0 ? cg_sk_groups(elem, cfg, &kp) : 0;
+    /* splits == 0: stream-K (forward and weight gradient) */
+    kp.sk = splits == 0 ? cg_sk_split(elem, cfg, &kp, &kp.sk_dp) : 0;
+    if (!kp.sk) kp.sk_dp = 0;
 
     /* workspace: repacked weights + split-K partials / stream-K pieces */
     size_t pbytes = kp.sk ? cg_sk_bytes(kp.sk, &t) : cg_partial_bytes(&kp);
@@ -694,9 +712,10 @@ static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, 
         c.B = nb;
         c.P = (long long)nb * ho * wo;
         cg_fill_tiles(&c, &t, kp.splits);
-        if (kp.sk) { /* as many workgroups as this chunk has units, at most */
-            long long units = (long long)c.mtiles * c.ntiles * c.KT;
-            c.sk = units < kp.sk ? (int)units : kp.sk;
+        if (kp.sk) { /* this chunk's own schedule, within the pieces the workspace holds */
+            c.sk = cg_sk_split(elem, cfg, &c, &c.sk_dp);
+            if (c.sk > kp.sk) c.sk = kp.sk;
+            if (!c.sk) c.sk_dp = 0;
         }
         c.A = (const char *)x + (size_t)b0 * img_a;
         c.a_bytes = (long long)nb * img_a;

@@ -109,9 +109,10 @@ int cg_features(int elem, const cg_kparams *kp0, int cfg, int splits, float *f) 
     f[26] = t.lp ? 1.0f : 0.0f;
     f[28] = lg(t.lp ? (double)t.WM * t.WN : 4.0); /* waves per workgroup (4 or 8) */
     f[29] = (float)t.lp;                           /* LDS stages of the pipelined kernels */
-    if (splits == 0) { /* stream-K: the persistent workgroups, no split partials */
-        const int g = cg_sk_groups(elem, cfg, &kp);
-        f[22] = lg((double)g);
+    if (splits == 0) { /* stream-K: data-parallel tiles + stream-K workgroups, no split partials */
+        int dp;
+        const int g = cg_sk_split(elem, cfg, &kp, &dp);
+        f[22] = lg((double)g + dp);
         f[24] = 0.0f;
         f[27] = 1.0f;
     }

@@ -174,10 +174,9 @@ int cg_candidates(int elem, const cg_kparams *base, int *cfgs, int *splits, int 
                 n++;
             }
         }
-        /* stream-K (splits 0) when the tiles do not fill the resident slots evenly */
+        /* stream-K (splits 0) when the tiles leave a partial round over the CUs */
         int g = cg_sk_groups(elem, c, &kp);
-        if (g > 0 && (long long)kp.mtiles * kp.ntiles % g != 0 && kp.KT >= 2 &&
-            cg_sk_bytes(g, &t) <= (size_t)CG_PACK_BYTES && n < max) {
+        if (g > 0 && kp.KT >= 2 && cg_sk_bytes(g, &t) <= (size_t)CG_PACK_BYTES && n < max) {
             cfgs[n] = c;
             splits[n] = 0;
             n++;

@@ -78,22 +78,42 @@ __global__ void __launch_bounds__(256) splitk_reduce_grouped_kernel(const cg_kpa
     }
 }
 
-// ---- stream-K fix-up: one workgroup per tile; tiles that several stream-K
-// workgroups share sum their pieces (workgroup order, deterministic) and go
-// through the epilogue; whole tiles were written back by their workgroup ----
+// ---- stream-K fix-up: one workgroup per tile of the stream-K region (tiles
+// sk_dp ..); tiles that several stream-K workgroups share sum their pieces
+// (workgroup order, deterministic) and go through the epilogue; whole tiles
+// were written back by their workgroup.
+//
+// A tile has BM x BN = 2^lbm x 2^lbn elements (every tile of the tables is
+// at least 64 x 64, a multiple of 2048).  Pieces are [BM][BN] floats, except
+// for the NCHW forward pass ([BN][BM]: pixels contiguous, as the output).
+// Block b takes elements [2048 c, 2048 c + 2048) of stream-K tile b / (TE /
+// 2048), c = b mod (TE / 2048): each thread sums 8 elements over all pieces,
+// then (beta != 0) loads their old values, then stores the 8, so no store
+// sits between loads that could overlap.  The per-channel epilogue vectors
+// of the tile's columns are staged in LDS. ----
+constexpr int FIX_ELEMS = 8 * 256;  // elements per fix-up block
+
 template <int OP>
-__global__ void __launch_bounds__(256) sk_fixup_kernel(const cg_kparams p, int BM, int BN) {
+__global__ void __launch_bounds__(256) sk_fixup_kernel(const cg_kparams p, int lbm, int lbn) {
     constexpr bool WG = OP == CG_WGRAD_NHWC || OP == CG_WGRAD_NCHW;
-    const int t = blockIdx.x;
+    constexpr bool FW = OP == CG_FWD_NHWC || OP == CG_FWD_NCHW;
+    constexpr bool PM = OP == CG_FWD_NCHW;  // pixel-major pieces
+    __shared__ float s_epi[5 * 256];
+    const int BM = 1 << lbm, BN = 1 << lbn, TE = BM * BN;
+    const int tid = threadIdx.x;
+    const int per_tile = TE / FIX_ELEMS;
+    const int tl = (int)blockIdx.x / per_tile;  // stream-K tile of this block
+    const int base = ((int)blockIdx.x - tl * per_tile) * FIX_ELEMS + tid;
+    const int t = p.sk_dp + tl;
     const long long G = p.sk;
-    const long long U = (long long)p.mtiles * p.ntiles * p.KT;
-    const long long lo = (long long)t * p.KT, hi = lo + p.KT;
+    const long long U = ((long long)p.mtiles * p.ntiles - p.sk_dp) * p.KT;
+    const long long lo = (long long)tl * p.KT, hi = lo + p.KT;
     auto first_unit = [&](long long g) { return g * U / G; };
     long long g0 = lo * G / U;
     while (g0 + 1 < G && first_unit(g0 + 1) <= lo) g0++;
     long long g1 = (hi - 1) * G / U;
     while (g1 + 1 < G && first_unit(g1 + 1) <= hi - 1) g1++;
-    if (g0 == g1) return;
+    if (g0 == g1) return;  // uniform: one workgroup wrote the whole tile
     const Ctx x = make_ctx(p, 0);
     const int nt = t % p.ntiles, mt = t / p.ntiles;
     int wg_tap = 0, wg_c0 = 0;
@@ -101,28 +121,68 @@ __global__ void __launch_bounds__(256) sk_fixup_kernel(const cg_kparams p, int B
         wg_tap = mt / p.kchunks;
         wg_c0 = (mt - wg_tap * p.kchunks) * BM;
     }
-    const long long tile_elems = (long long)BM * BN;
-    for (int e = threadIdx.x; e < BM * BN; e += blockDim.x) {
-        const int ml = e / BN, nl = e - ml * BN;
-        const int n = nt * BN + nl;
-        int m;
-        bool ok;
+    const bool staged = epi_staged<OP>(p) && BN <= 256;
+    if (staged) {
+        epi_stage(p, nt * BN, BN, s_epi, tid, 256);
+        __syncthreads();
+    }
+    auto coords = [&](int e, int &m, int &n, int &nl) __attribute__((always_inline)) {
+        int ml;
+        if (PM) {
+            nl = e >> lbm;
+            ml = e & (BM - 1);
+        } else {
+            ml = e >> lbn;
+            nl = e & (BN - 1);
+        }
+        n = nt * BN + nl;
         if (WG) {
             const int c = wg_c0 + ml;
             m = wg_tap * p.C + c;
-            ok = c < p.C && n < p.KN;
-        } else {
-            m = mt * BM + ml;
-            ok = m < x.Mrows && n < p.KN;
+            return c < p.C && n < p.KN;
         }
-        if (!ok) continue;
-        float s = 0.f;
+        m = mt * BM + ml;
+        return m < x.Mrows && n < p.KN;
+    };
+    {
+        float acc[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc[j] = 0.f;
         for (long long g = g0; g <= g1; g++) {
             const long long f = first_unit(g);
             if (first_unit(g + 1) == f) continue;  // an empty range wrote nothing
-            s += p.ws[(g * 2 + (f >= lo ? 0 : 1)) * tile_elems + e];
+            const float *pc = p.ws + (g * 2 + (f >= lo ? 0 : 1)) * (long long)TE + base;
+#pragma unroll
+            for (int j = 0; j < 8; j++) acc[j] += pc[j * 256];
         }
-        store_out<OP>(p, x, m, n, s);
+        float old[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            int m, n, nl;
+            const bool ok = coords(base + j * 256, m, n, nl);
+            old[j] = (p.beta != 0.0f && ok) ? p.out[out_addr<OP>(p, x, m, n)] : 0.f;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            int m, n, nl;
+            if (!coords(base + j * 256, m, n, nl)) continue;
+            float ev[5] = {0.f, 0.f, 1.f, 1.f, 0.f};
+            if (FW) {
+                if (staged) {
+#pragma unroll
+                    for (int k = 0; k < 5; k++) ev[k] = s_epi[k * BN + nl];
+                } else {
+                    if (p.bias) ev[0] = p.bias[n];
+                    if (p.mean) {
+                        ev[1] = p.mean[n];
+                        ev[2] = p.inv_std[n];
+                        ev[3] = p.gamma[n];
+                        ev[4] = p.bnb[n];
+                    }
+                }
+            }
+            p.out[out_addr<OP>(p, x, m, n)] = epi_value<OP>(p, acc[j], old[j], ev);
+        }
     }
 }
 
@@ -240,19 +300,23 @@ extern "C" int cg_igemm_sk_occupancy(int elem, int cfg, const cg_kparams *kp0) {
 
 extern "C" int cg_launch_sk_fixup(const cg_kparams *kp, int BM, int BN, void *stream) {
     hipStream_t st = (hipStream_t)stream;
-    const unsigned tiles = (unsigned)(kp->mtiles * kp->ntiles);
-    if (tiles == 0) return 0;
+    const int tiles_all = kp->mtiles * kp->ntiles;
+    if (tiles_all <= kp->sk_dp) return 0;
+    const unsigned tiles = (unsigned)(tiles_all - kp->sk_dp);  // the stream-K region
+    int lbm = 0, lbn = 0;
+    while ((1 << lbm) < BM) lbm++;
+    while ((1 << lbn) < BN) lbn++;
+    if ((1 << lbm) != BM || (1 << lbn) != BN || BM * BN < FIX_ELEMS) return (int)hipErrorInvalidValue;
+    const unsigned blocks = tiles * (unsigned)(BM * BN / FIX_ELEMS);
+#define CG_FX(OPK) hipLaunchKernelGGL(sk_fixup_kernel<OPK>, dim3(blocks), dim3(256), 0, st, *kp, lbm, lbn)
     switch (kp->op) {
-    case CG_FWD_NHWC: hipLaunchKernelGGL(sk_fixup_kernel<CG_FWD_NHWC>, dim3(tiles), dim3(256), 0, st, *kp, BM, BN); break;
-    case CG_FWD_NCHW: hipLaunchKernelGGL(sk_fixup_kernel<CG_FWD_NCHW>, dim3(tiles), dim3(256), 0, st, *kp, BM, BN); break;
-    case CG_WGRAD_NHWC:
-        hipLaunchKernelGGL(sk_fixup_kernel<CG_WGRAD_NHWC>, dim3(tiles), dim3(256), 0, st, *kp, BM, BN);
-        break;
-    case CG_WGRAD_NCHW:
-        hipLaunchKernelGGL(sk_fixup_kernel<CG_WGRAD_NCHW>, dim3(tiles), dim3(256), 0, st, *kp, BM, BN);
-        break;
+    case CG_FWD_NHWC: CG_FX(CG_FWD_NHWC); break;
+    case CG_FWD_NCHW: CG_FX(CG_FWD_NCHW); break;
+    case CG_WGRAD_NHWC: CG_FX(CG_WGRAD_NHWC); break;
+    case CG_WGRAD_NCHW: CG_FX(CG_WGRAD_NCHW); break;
     default: return (int)hipErrorInvalidValue;
     }
+#undef CG_FX
     return (int)hipGetLastError();
 }
 

@@ -267,4 +267,77 @@ __device__ __attribute__((always_inline)) inline void store_block(const cg_kpara
     }
 }
 
+// ---- stream-K schedule (blocks of a *_sk_kernel) ----
+// Units are (tile, K step) pairs numbered tile * KT + k.  Block b < sk_dp
+// owns the units of data-parallel tile xcd_run(b, sk_dp); block sk_dp + i
+// owns units [sk_dp*KT + g*U/G, sk_dp*KT + (g+1)*U/G), g = xcd_run(i, G),
+// U = (tiles - sk_dp)*KT, G = p.sk.
+//
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8);
+// xcd_run gives each XCD a contiguous run of the index space, so neighbouring
+// tiles (shared weight panel, shared input halo) meet in one L2.
+__device__ inline int xcd_run(int bid, int nb) {
+    const int xcd = bid & 7, per = nb >> 3, rem = nb & 7;
+    return xcd * per + min(xcd, rem) + (bid >> 3);
+}
+
+// The kernel's cg_kparams argument read through the kernarg segment pointer
+// (loads from constant memory where they are used) instead of as values
+// loaded at the kernel entry and kept live across the whole body.
+// The argument must be the kernel's first (only) parameter, which sits at
+// the start of the kernarg segment.  The host pass of the compiler only
+// parses this device function.
+__device__ inline const cg_kparams &kernarg_params(const cg_kparams &arg) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    (void)arg;
+    return *reinterpret_cast<const cg_kparams *>(__builtin_amdgcn_kernarg_segment_ptr());
+#else
+    return arg;
+#endif
+}
+
+struct SkRange {
+    long long u, u_first, u_end;
+    int g;  // stream-K workgroup index (-1: a data-parallel block)
+};
+
+struct SkPiece {
+    int tile, k0, k1;
+    float *piece;  // nullptr: the whole K range, written back directly
+};
+
+__device__ inline SkRange sk_range(const cg_kparams &p) {
+    SkRange r;
+    const int b = (int)blockIdx.x;
+    if (b < p.sk_dp) {
+        r.g = -1;
+        r.u_first = (long long)xcd_run(b, p.sk_dp) * p.KT;
+        r.u_end = r.u_first + p.KT;
+    } else {
+        r.g = xcd_run(b - p.sk_dp, p.sk);
+        const long long base = (long long)p.sk_dp * p.KT;
+        const long long U = ((long long)p.mtiles * p.ntiles - p.sk_dp) * p.KT;
+        r.u_first = base + (long long)r.g * U / p.sk;
+        r.u_end = base + (long long)(r.g + 1) * U / p.sk;
+    }
+    r.u = r.u_first;
+    return r;
+}
+
+// The next piece of the range (false when it is exhausted).  A piece that is
+// not a whole tile goes to the workgroup's slot 0 (its first piece) or 1.
+__device__ inline bool sk_next(const cg_kparams &p, SkRange &r, SkPiece &q, int tile_elems) {
+    if (r.u >= r.u_end) return false;
+    const int t = (int)(r.u / p.KT);
+    const int k0 = (int)(r.u - (long long)t * p.KT);
+    const int k1 = (int)min((long long)p.KT, (long long)k0 + (r.u_end - r.u));
+    q.tile = t;
+    q.k0 = k0;
+    q.k1 = k1;
+    q.piece = (k0 == 0 && k1 == p.KT) ? nullptr
+                                      : p.ws + ((long long)r.g * 2 + (r.u == r.u_first ? 0 : 1)) * tile_elems;
+    r.u += k1 - k0;
+    return true;
+}
+
 }  // namespace cg

@@ -175,7 +175,7 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
     // once and the small weight panel stays L2-resident (with M fastest the
     // gathered operand was measured re-read once per N tile).  Backward data:
     // the residue classes of one tile are adjacent items, so every XCD's run
-    // of items (xcd_remap) holds all classes in equal shares -- with the class
+    // of items (xcd_run) holds all classes in equal shares -- with the class
     // outermost, each XCD ran one class and the XCDs with the most taps per
     // class set the time -- and within a tile the class with the most taps
     // comes first (cls_order).
@@ -624,8 +624,9 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
                     const int ir = roff + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
                     const int ic = coff + b * 32 + (lane & 31);
                     const int ml = SW ? ic : ir, nl = SW ? ir : ic;
-                    if (piece) {
-                        piece[ml * BN + nl] = acc[a][b][v];
+                    if (piece) {  // [BM][BN]; pixel-major [BN][BM] for the NCHW forward (as its output)
+                        if constexpr (OP == CG_FWD_NCHW) piece[nl * BM + ml] = acc[a][b][v];
+                        else piece[ml * BN + nl] = acc[a][b][v];
                         continue;
                     }
                     int m;
@@ -722,40 +723,23 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
     if (s_epi) __syncthreads();  // the LDS is rewritten by the next item of a persistent launch
 }
 
-// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8);
-// give each XCD a contiguous run of the item space so neighbouring tiles
-// (shared weight panel, shared input halo) meet in one L2.
-__device__ inline int xcd_remap(int bid, int nb) {
-    const int xcd = bid & 7, per = nb >> 3, rem = nb & 7;
-    return xcd * per + min(xcd, rem) + (bid >> 3);
-}
 
 template <typename T, int BM, int BN, int BK, int WM, int WN, int OP, bool VEC>
 __global__ void __launch_bounds__(NT) igemm_kernel(const cg_kparams p) {
-    igemm_tile<T, BM, BN, BK, WM, WN, OP, VEC>(p, xcd_remap(blockIdx.x, gridDim.x));
+    igemm_tile<T, BM, BN, BK, WM, WN, OP, VEC>(p, xcd_run(blockIdx.x, gridDim.x));
 }
 
-// Stream-K (FWD/WGRAD): the tiles x K-tiles of the problem are cut into
-// gridDim.x equal contiguous ranges, one per persistent workgroup (ranges of
-// one XCD adjacent).  A workgroup reduces the K range of each tile its range
-// covers; a whole tile is written back directly, a partial one (only the
-// first and last of a range can be) goes to the workgroup's piece 0 / 1 and
-// sk_fixup_kernel sums the pieces of every shared tile in workgroup order.
+// Stream-K (FWD/WGRAD), with the schedule of lp_sk_kernel (sk_range /
+// sk_next): sk_dp data-parallel tiles, then p.sk workgroups sharing the K
+// tiles of the remaining tiles; the pieces of shared tiles are summed by
+// sk_fixup_kernel in workgroup order.
 template <typename T, int BM, int BN, int BK, int WM, int WN, int OP, bool VEC>
-__global__ void __launch_bounds__(NT) igemm_sk_kernel(const cg_kparams p) {
-    const int G = gridDim.x;
-    const int g = xcd_remap(blockIdx.x, G);
-    const long long U = (long long)p.mtiles * p.ntiles * p.KT;
-    const long long u_first = (long long)g * U / G, u_end = (long long)(g + 1) * U / G;
-    for (long long u = u_first; u < u_end;) {
-        const int t = (int)(u / p.KT);
-        const int k0 = (int)(u - (long long)t * p.KT);
-        const int k1 = (int)min((long long)p.KT, (long long)k0 + (u_end - u));
-        float *piece = (k0 == 0 && k1 == p.KT) ? nullptr
-                                                : p.ws + ((long long)g * 2 + (u == u_first ? 0 : 1)) * (BM * BN);
-        igemm_tile<T, BM, BN, BK, WM, WN, OP, VEC>(p, t, k0, k1, piece);
-        u += k1 - k0;
-        __syncthreads();  // LDS of this tile is free before the next one starts
+__global__ void __launch_bounds__(NT) igemm_sk_kernel(const cg_kparams p_arg) {
+    const cg_kparams &p = kernarg_params(p_arg);
+    SkRange r = sk_range(p);
+    for (SkPiece q; sk_next(p, r, q, BM * BN);) {
+        igemm_tile<T, BM, BN, BK, WM, WN, OP, VEC>(p, q.tile, q.k0, q.k1, q.piece);
+        __syncthreads();  // LDS of this piece is free before the next one starts
     }
 }
 
@@ -779,7 +763,7 @@ __global__ void __launch_bounds__(NT) igemm_grouped_kernel(const cg_kparams *__r
             else hi = mid - 1;
         }
         const cg_kparams &p = plist[lo];
-        const int lin = xcd_remap(item - start[lo], start[lo + 1] - start[lo]);
+        const int lin = xcd_run(item - start[lo], start[lo + 1] - start[lo]);
         const int count = p.mtiles * p.ntiles * p.splits * p.ncls_h * p.ncls_w;
         __syncthreads();  // LDS reads of the previous item are complete
         if (lin < count) igemm_tile<T, BM, BN, BK, WM, WN, OP, VEC>(p, lin);
@@ -821,8 +805,8 @@ inline int launch_igemm(const cg_kparams *kp, const cg_group *g, int *occ, hipSt
         if constexpr (!SKOK) {
             return (int)hipErrorInvalidValue;
         } else {
-            hipLaunchKernelGGL((igemm_sk_kernel<T, BM, BN, BK, WM, WN, OP, VEC>), dim3((unsigned)kp->sk), dim3(NT),
-                               0, st, *kp);
+            hipLaunchKernelGGL((igemm_sk_kernel<T, BM, BN, BK, WM, WN, OP, VEC>),
+                               dim3((unsigned)(kp->sk_dp + kp->sk)), dim3(NT), 0, st, *kp);
             return (int)hipGetLastError();
         }
     }

@@ -400,36 +400,32 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
     if (s_epi) __syncthreads();  // the LDS is rewritten by the next item of a persistent launch
 }
 
-__device__ inline int lp_xcd_remap(int bid, int nb) {
-    const int xcd = bid & 7, per = nb >> 3, rem = nb & 7;
-    return xcd * per + min(xcd, rem) + (bid >> 3);
-}
-
 template <typename T, int BM, int BN, int WM, int WN, int NS>
 __global__ void __launch_bounds__(WM * WN * 64) lp_kernel(const cg_kparams p) {
-    lp_tile<T, BM, BN, WM, WN, NS>(p, lp_xcd_remap(blockIdx.x, gridDim.x));
+    lp_tile<T, BM, BN, WM, WN, NS>(p, xcd_run(blockIdx.x, gridDim.x));
 }
 
-// Stream-K: the tiles x K steps of the problem are cut into gridDim.x equal
-// contiguous ranges, one per persistent workgroup.  A whole tile is written
-// back directly; a partial one (only the first and last of a range can be)
-// goes to the workgroup's piece 0 / 1, and sk_fixup_kernel sums the pieces
-// of every shared tile in workgroup order.
+// Stream-K: blocks 0 .. sk_dp-1 each run one whole tile of the first sk_dp
+// (data-parallel); the K steps of the remaining tiles are cut into p.sk equal
+// contiguous ranges, one per later block.  A whole tile is written back
+// directly; a partial one (only the first and last of a range can be) goes
+// to the workgroup's piece 0 / 1, and sk_fixup_kernel sums the pieces of
+// every shared tile in workgroup order.  sk_dp is a multiple of 8, so the XCD
+// of a block of either region is its index within the region mod 8.
+//
+// Both regions run one loop with a single call of the tile body, and the
+// parameters are read through the kernarg pointer where used: with a second
+// inlined copy of the body and the parameters held as values from the kernel
+// entry, the scalar state live across the body overflowed the SGPRs and the
+// spills were reloaded with v_readlane all through the K loop (2580 reloads
+// in the 64x128 NCHW kernel, against 21 now).
 template <typename T, int BM, int BN, int WM, int WN, int NS>
-__global__ void __launch_bounds__(WM * WN * 64) lp_sk_kernel(const cg_kparams p) {
-    const int G = gridDim.x;
-    const int g = lp_xcd_remap(blockIdx.x, G);
-    const long long U = (long long)p.mtiles * p.ntiles * p.KT;
-    const long long u_first = (long long)g * U / G, u_end = (long long)(g + 1) * U / G;
-    for (long long u = u_first; u < u_end;) {
-        const int t = (int)(u / p.KT);
-        const int k0 = (int)(u - (long long)t * p.KT);
-        const int k1 = (int)min((long long)p.KT, (long long)k0 + (u_end - u));
-        float *piece = (k0 == 0 && k1 == p.KT) ? nullptr
-                                                : p.ws + ((long long)g * 2 + (u == u_first ? 0 : 1)) * (BM * BN);
-        lp_tile<T, BM, BN, WM, WN, NS>(p, t, k0, k1, piece);
-        u += k1 - k0;
-        __syncthreads();  // loads, LDS reads and stores of this tile are complete
+__global__ void __launch_bounds__(WM * WN * 64) lp_sk_kernel(const cg_kparams p_arg) {
+    const cg_kparams &p = kernarg_params(p_arg);
+    SkRange r = sk_range(p);
+    for (SkPiece q; sk_next(p, r, q, BM * BN);) {
+        lp_tile<T, BM, BN, WM, WN, NS>(p, q.tile, q.k0, q.k1, q.piece);
+        __syncthreads();  // loads, LDS reads and stores of this piece are complete
     }
 }
 
@@ -445,7 +441,7 @@ __global__ void __launch_bounds__(WM * WN * 64) lp_grouped_kernel(const cg_kpara
             else hi = mid - 1;
         }
         const cg_kparams &p = plist[lo];
-        const int lin = lp_xcd_remap(item - start[lo], start[lo + 1] - start[lo]);
+        const int lin = xcd_run(item - start[lo], start[lo + 1] - start[lo]);
         const int count = p.mtiles * p.ntiles * p.splits;
         __syncthreads();  // every LDS read and load of the previous item is complete
         if (lin < count) lp_tile<T, BM, BN, WM, WN, NS>(p, lin);
@@ -463,7 +459,8 @@ inline int launch_lp(const cg_kparams *kp, const cg_group *g, int *occ, hipStrea
         return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k, NT, 0);
     }
     if (kp->sk > 0 && !g) {
-        hipLaunchKernelGGL((lp_sk_kernel<T, BM, BN, WM, WN, NS>), dim3((unsigned)kp->sk), dim3(NT), 0, st, *kp);
+        hipLaunchKernelGGL((lp_sk_kernel<T, BM, BN, WM, WN, NS>), dim3((unsigned)(kp->sk_dp + kp->sk)), dim3(NT), 0,
+                           st, *kp);
         return (int)hipGetLastError();
     }
     if (g) {

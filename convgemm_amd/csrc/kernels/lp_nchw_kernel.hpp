@@ -285,8 +285,8 @@ __device__ __attribute__((always_inline)) inline void lpn_tile(const cg_kparams 
 #pragma unroll
                 for (int v = 0; v < 16; v++) {
                     const int nl = wn * WTN + a * 32 + 8 * (v >> 2) + 4 * h + (v & 3), ml = wm * WTM + b * 32 + l32;
-                    if (piece) {
-                        piece[ml * BN + nl] = acc[a][b][v];
+                    if (piece) {  // pixel-major, as the output: lanes store consecutive floats
+                        piece[nl * BM + ml] = acc[a][b][v];
                         continue;
                     }
                     const int n = nt * BN + nl, m = mt * BM + ml;
@@ -332,29 +332,18 @@ __device__ __attribute__((always_inline)) inline void lpn_tile(const cg_kparams 
 
 template <int BM, int BN, int WM, int WN, int NS>
 __global__ void __launch_bounds__(LP_NT) lpn_kernel(const cg_kparams p) {
-    lpn_tile<BM, BN, WM, WN, NS>(p, lp_xcd_remap(blockIdx.x, gridDim.x));
+    lpn_tile<BM, BN, WM, WN, NS>(p, xcd_run(blockIdx.x, gridDim.x));
 }
 
-// Stream-K: the tiles x K steps of the problem are cut into gridDim.x equal
-// contiguous ranges, one per persistent workgroup.  A whole tile is written
-// back directly; a partial one (only the first and last of a range can be)
-// goes to the workgroup's piece 0 / 1, and sk_fixup_kernel sums the pieces
-// of every shared tile in workgroup order.
+// Stream-K, with the schedule of lp_sk_kernel: sk_dp data-parallel tiles,
+// then p.sk workgroups sharing the K steps of the remaining tiles.
 template <int BM, int BN, int WM, int WN, int NS>
-__global__ void __launch_bounds__(LP_NT) lpn_sk_kernel(const cg_kparams p) {
-    const int G = gridDim.x;
-    const int g = lp_xcd_remap(blockIdx.x, G);
-    const long long U = (long long)p.mtiles * p.ntiles * p.KT;
-    const long long u_first = (long long)g * U / G, u_end = (long long)(g + 1) * U / G;
-    for (long long u = u_first; u < u_end;) {
-        const int t = (int)(u / p.KT);
-        const int k0 = (int)(u - (long long)t * p.KT);
-        const int k1 = (int)min((long long)p.KT, (long long)k0 + (u_end - u));
-        float *piece = (k0 == 0 && k1 == p.KT) ? nullptr
-                                                : p.ws + ((long long)g * 2 + (u == u_first ? 0 : 1)) * (BM * BN);
-        lpn_tile<BM, BN, WM, WN, NS>(p, t, k0, k1, piece);
-        u += k1 - k0;
-        __syncthreads();  // loads, LDS reads and stores of this tile are complete
+__global__ void __launch_bounds__(LP_NT) lpn_sk_kernel(const cg_kparams p_arg) {
+    const cg_kparams &p = kernarg_params(p_arg);
+    SkRange r = sk_range(p);
+    for (SkPiece q; sk_next(p, r, q, BM * BN);) {
+        lpn_tile<BM, BN, WM, WN, NS>(p, q.tile, q.k0, q.k1, q.piece);
+        __syncthreads();  // loads, LDS reads and stores of this piece are complete
     }
 }
 
@@ -369,7 +358,7 @@ __global__ void __launch_bounds__(LP_NT) lpn_grouped_kernel(const cg_kparams *__
             else hi = mid - 1;
         }
         const cg_kparams &p = plist[lo];
-        const int lin = lp_xcd_remap(item - start[lo], start[lo + 1] - start[lo]);
+        const int lin = xcd_run(item - start[lo], start[lo + 1] - start[lo]);
         const int count = p.mtiles * p.ntiles * p.splits;
         __syncthreads();  // every LDS read and load of the previous item is complete
         if (lin < count) lpn_tile<BM, BN, WM, WN, NS>(p, lin);
@@ -386,7 +375,8 @@ inline int launch_lpn(const cg_kparams *kp, const cg_group *g, int *occ, hipStre
         return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k, LP_NT, 0);
     }
     if (kp->sk > 0 && !g) {
-        hipLaunchKernelGGL((lpn_sk_kernel<BM, BN, WM, WN, NS>), dim3((unsigned)kp->sk), dim3(LP_NT), 0, st, *kp);
+        hipLaunchKernelGGL((lpn_sk_kernel<BM, BN, WM, WN, NS>), dim3((unsigned)(kp->sk_dp + kp->sk)), dim3(LP_NT), 0,
+                           st, *kp);
         return (int)hipGetLastError();
     }
     if (g) {

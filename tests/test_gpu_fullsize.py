@@ -98,8 +98,9 @@ def _gather(layout, Y, ys):
 @pytest.mark.parametrize("name", list(LAYERS))
 def test_fullsize_pipelined_forward(name, layout):
     """Every pipelined FP32 forward configuration of this layout, forced on the
-    full-size layer with the fused epilogue (split-K 1 and 3): 256 sampled
-    outputs against the FP64 oracle."""
+    full-size layer with the fused epilogue (split-K 1 and 3, and stream-K,
+    which on these tile counts runs data-parallel tiles and a stream-K
+    remainder): 256 sampled outputs against the FP64 oracle."""
     s = LAYERS[name]
     cfgs = [i for i, t in enumerate(cg.tile_configs(cg.F32))
             if t[5] > 0 and cg.tile_supports(cg.F32, i, cg.FWD, layout)]
@@ -122,7 +123,7 @@ def test_fullsize_pipelined_forward(name, layout):
                      s.vdil, s.hdil)
     Y = torch.empty(s.y_shape(layout), device="cuda")
     for cfg in cfgs:
-        for splits in (1, 3):
+        for splits in (1, 3, 0):  # 0: stream-K (data-parallel tiles + a stream-K remainder)
             print(f"full-size pipelined {name} layout={layout} cfg {cfg} splits {splits}", flush=True)
             Y.fill_(float("nan"))
             cg.force_config(cfg, splits)
@@ -134,3 +135,44 @@ def test_fullsize_pipelined_forward(name, layout):
                 cg.force_config(-1, 0)
             got = _gather(layout, Y, ys)
             assert np.max(np.abs(got - ref)) <= TOL * max(np.max(np.abs(ref)), 1.0), (cfg, splits)
+
+
+@pytest.mark.parametrize("layout", [NHWC, NCHW])
+@pytest.mark.parametrize("name", ["L1_1x1_256to64_56", "L3_3x3s2_128to128_56", "L4_1x1_1024to256_14"])
+def test_fullsize_stream_k_general_kernel(name, layout):
+    """Stream-K on the general kernel (forward with the fused epilogue, and the
+    weight gradient), forced on full-size layers whose 64x64 tile counts are
+    not multiples of the 256 CUs, so data-parallel tiles, stream-K pieces and
+    the fix-up all run: sampled outputs against the FP64 oracle."""
+    s = LAYERS[name]
+    cfg = next(i for i, t in enumerate(cg.tile_configs(cg.F32)) if t[:2] == (64, 64) and t[2] == 32 and t[5] == 0)
+    g = torch.Generator(device="cuda").manual_seed(sum(map(ord, name)) * 5 + layout)
+    X = torch.randn(s.x_shape(layout), device="cuda", generator=g)
+    W = torch.randn(s.w_shape(layout), device="cuda", generator=g) * (2.0 / s.K) ** 0.5
+    dY = torch.randn(s.y_shape(layout), device="cuda", generator=g)
+    bias = 0.1 * torch.randn(s.kn, device="cuda", generator=g)
+    geo = (s.b, s.h, s.w, s.c, s.kn, s.kh, s.kw, s.vpad, s.hpad, s.vstr, s.hstr, s.vdil, s.hdil)
+    Y = torch.full(s.y_shape(layout), float("nan"), device="cuda")
+    dW = torch.full(s.w_shape(layout), float("nan"), device="cuda")
+    print(f"stream-K general kernel {name} layout={layout} cfg {cfg}", flush=True)
+    cg.force_config(cfg, 0)
+    try:
+        cg.conv_ex(cg.make_desc(cg.FWD, layout, *geo), W, X, Y, bias=bias, relu=True)
+        torch.cuda.synchronize()
+        plan = cg.last_plan()  # (a weight repack), the stream-K kernel, the fix-up
+        assert plan[:2] == (cfg, 0) and plan[2] >= 2, plan
+        cg.conv_ex(cg.make_desc(cg.WGRAD, layout, *geo), dY, X, dW)
+        torch.cuda.synchronize()
+        assert cg.last_plan()[:2] == (cfg, 0), cg.last_plan()
+    finally:
+        cg.force_config(-1, 0)
+    rng = np.random.default_rng(13)
+    Xh = X.cpu().numpy()
+    ys = np.stack([rng.integers(0, n, 256) for n in (s.b, s.ho, s.wo, s.kn)], 1)
+    ref = sampled.fwd_sampled(layout, s, W.cpu().numpy(), Xh, ys, bias=bias.cpu().numpy(), relu=True)
+    got = _gather(layout, Y, ys)
+    assert np.max(np.abs(got - ref)) <= TOL * max(np.max(np.abs(ref)), 1.0)
+    ws = np.stack([rng.integers(0, n, 48) for n in (s.c, s.kh, s.kw, s.kn)], 1)
+    ref = sampled.wgrad_sampled(layout, s, dY.cpu().numpy(), Xh, ws)
+    got = sampled.w_index(layout, dW.cpu().numpy(), ws)
+    assert np.max(np.abs(got - ref)) <= TOL * np.max(np.abs(ref))
