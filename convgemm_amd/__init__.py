@@ -61,7 +61,7 @@ EXPORTED_SYMBOLS = (
 
 PLAN_ANALYTIC, PLAN_TUNED, PLAN_LEARNED, PLAN_MODEL = 0, 1, 2, 3
 # CONVGEMM_MAX_CANDIDATES of convgemm_tune.h: candidate plans of one problem at most
-MAX_CANDIDATES = 1024
+MAX_CANDIDATES = 2048
 
 
 class ConvGemmError(RuntimeError):
@@ -209,7 +209,9 @@ def lib() -> ctypes.CDLL:
     L.cg_split_factors.restype = ci
     L.cg_num_tilecfgs.argtypes = [ci]
     L.cg_num_tilecfgs.restype = ci
-    L.cg_get_tilecfg.argtypes = [ci, ci, ctypes.POINTER(ci * 6)]
+    L.cg_get_tilecfg.argtypes = [ci, ci, ctypes.POINTER(ci * 7)]
+    L.cg_num_tile_shapes.argtypes = [ci]
+    L.cg_num_tile_shapes.restype = ci
     L.cg_get_tilecfg.restype = ci
     L.cg_tilecfg_supports.argtypes = [ci, ci, ci, ci]
     L.cg_tilecfg_supports.restype = ci
@@ -683,14 +685,22 @@ def weights_nchw_to_nhwc(src, dst, kn, c, kh, kw, esize=4, stream=None):
 
 
 def tile_configs(dtype=F32):
-    """Tile table of an operand type: list of (BM, BN, BK, WM, WN, pipelined stages or 0)."""
+    """Configurations of an operand type: list of (BM, BN, BK, WM, WN, pipelined stages or 0,
+    raster order).  Configuration base + r * num_tile_shapes(dtype) is tile shape `base`
+    walked in raster order r (0: the N panels of an M panel adjacent, 1: the M panels of an
+    N panel adjacent, 2: bands of 8 M panels)."""
     L = lib()
     out = []
     for c in range(L.cg_num_tilecfgs(int(dtype))):
-        v = (ctypes.c_int * 6)()
+        v = (ctypes.c_int * 7)()
         if L.cg_get_tilecfg(int(dtype), c, ctypes.byref(v)) == 0:
             out.append(tuple(v))
     return out
+
+
+def num_tile_shapes(dtype=F32) -> int:
+    """Tile shapes of an operand type (configurations 0 .. n-1 are the shapes in raster order 0)."""
+    return int(lib().cg_num_tile_shapes(int(dtype)))
 
 
 def dgrad_class_order(desc: Desc):

@@ -63,18 +63,26 @@ typedef struct {
     float alpha, beta;
     float scale;     /* product of operand scales (FP8), 1 otherwise */
     int sk;          /* stream-K workgroups (FWD/WGRAD, splits == 1); 0: off */
+    int raster;      /* order of the tiles (tile_mn): 0 N panels of an M panel adjacent, 1 M panels of an
+                        N panel adjacent, 2 bands of 8 M panels (M fastest within a band) */
     int sk_dp;       /* with sk: tiles 0 .. sk_dp-1 run one per workgroup (blocks 0 .. sk_dp-1), the
                         sk workgroups after them share the K steps of tiles sk_dp .. mtiles*ntiles-1 */
 } cg_kparams;
 
-/* Tile configurations of the implicit-GEMM kernels. */
+/* Tile configurations of the implicit-GEMM kernels: a tile shape and kernel
+ * (tile_tables.hpp) and the order its tiles are walked in.  Configuration
+ * base + r * (number of tile shapes) is tile shape `base` in raster order r,
+ * so order 0 keeps the tile shapes' own indices. */
 typedef struct {
     int BM, BN, BK;
-    int WM, WN;      /* waves along M and N (WM*WN = 4) */
+    int WM, WN;      /* waves along M and N (WM*WN = 4, or 8 for some pipelined tiles) */
     int lp;          /* > 0: a pipelined forward kernel (NHWC; NCHW for FP32) with lp LDS stages */
+    int raster;      /* tile order (cg_kparams.raster) */
 } cg_tilecfg;
+#define CG_NUM_RASTERS 3
 
-int cg_num_tilecfgs(int elem);
+int cg_num_tilecfgs(int elem);      /* tile shapes x raster orders */
+int cg_num_tile_shapes(int elem);
 int cg_get_tilecfg(int elem, int cfg, cg_tilecfg *out);
 /* Is (elem, cfg, op, vec) instantiated? */
 int cg_tilecfg_supports(int elem, int cfg, int op, int vec);

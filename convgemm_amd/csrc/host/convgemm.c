@@ -332,6 +332,7 @@ static unsigned long long dgrad_class_order(const cg_kparams *kp) {
 /* Fill the tile-space fields of kp for configuration t. */
 void cg_fill_tiles(cg_kparams *kp, const cg_tilecfg *t, int splits) {
     int op = kp->op;
+    kp->raster = t->raster;
     kp->ncls_h = 1;
     kp->ncls_w = 1;
     kp->cls_order = 0;
@@ -379,7 +380,7 @@ void cg_default_plan(int elem, const cg_kparams *base, int *cfg_out, int *splits
     int op = base->op;
     double best = 1e300;
     int best_cfg = -1, best_splits = 1;
-    int n = cg_num_tilecfgs(elem);
+    int n = cg_num_tile_shapes(elem); /* raster order 0: the N panels of an M panel adjacent */
     for (int cfg = 0; cfg < n; cfg++) {
         cg_tilecfg t;
         if (cg_get_tilecfg(elem, cfg, &t) || !cg_tilecfg_supports(elem, cfg, op, base->vec)) continue;

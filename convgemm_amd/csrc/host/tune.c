@@ -162,6 +162,11 @@ int cg_candidates(int elem, const cg_kparams *base, int *cfgs, int *splits, int 
         if (cg_get_tilecfg(elem, c, &t) || !cg_tilecfg_supports(elem, c, base->op, base->vec)) continue;
         cg_kparams kp = *base;
         cg_fill_tiles(&kp, &t, 1);
+        /* a raster order that walks the tiles as order 0 does (one M or one N
+         * panel), or as order 1 does (order 2 with at most one band), adds
+         * nothing */
+        if (t.raster > 0 && (kp.mtiles == 1 || kp.ntiles == 1)) continue;
+        if (t.raster == 2 && kp.mtiles <= 8) continue;
         for (unsigned si = 0; si < sizeof(S) / sizeof(S[0]); si++) {
             int s = S[si];
             if (s > 1 && kp.KT / s < 2) break;
@@ -369,7 +374,7 @@ convgemm_status_t convgemm_tune_load(const char *path) {
     FILE *f = fopen(path, "r");
     if (!f) return cg_set_error(CONVGEMM_ERR_INVALID_VALUE, "cannot open tuning database");
     /* a line holds at most CONVGEMM_MAX_CANDIDATES triples of at most 48 characters */
-    const int line_max = 1 << 16;
+    const int line_max = 48 * CONVGEMM_MAX_CANDIDATES + 256;
     char *line = malloc(line_max);
     if (!line) {
         fclose(f);

@@ -115,7 +115,8 @@ __global__ void __launch_bounds__(256) sk_fixup_kernel(const cg_kparams p, int l
     while (g1 + 1 < G && first_unit(g1 + 1) <= hi - 1) g1++;
     if (g0 == g1) return;  // uniform: one workgroup wrote the whole tile
     const Ctx x = make_ctx(p, 0);
-    const int nt = t % p.ntiles, mt = t / p.ntiles;
+    int mt, nt;
+    tile_mn(p, t, mt, nt);
     int wg_tap = 0, wg_c0 = 0;
     if (WG) {
         wg_tap = mt / p.kchunks;

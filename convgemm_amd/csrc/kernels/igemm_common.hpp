@@ -267,6 +267,33 @@ __device__ __attribute__((always_inline)) inline void store_block(const cg_kpara
     }
 }
 
+// Tile t -> (M panel mt, N panel nt) in the problem's raster order:
+//   0: the N panels of one M panel adjacent (t = mt * ntiles + nt): the
+//      gathered operand's panel is fetched once and the small weight panel
+//      stays L2-resident;
+//   1: the M panels of one N panel adjacent (t = nt * mtiles + mt): the
+//      weight panel is fetched once per run of M panels (the reference's
+//      A3B2C0 order, src/gemm_blis_A3B2C0.c:55-80, puts the B panel outside
+//      its ic loop likewise);
+//   2: bands of 8 M panels, the M panels of one N panel adjacent within a
+//      band: an XCD's run of tiles touches 8 M panels and their N panels.
+__device__ inline void tile_mn(const cg_kparams &p, int t, int &mt, int &nt) {
+    if (p.raster == 1) {
+        nt = t / p.mtiles;
+        mt = t - nt * p.mtiles;
+    } else if (p.raster == 2) {
+        const int band = t / (8 * p.ntiles);
+        const int m0 = band * 8;
+        const int bh = min(8, p.mtiles - m0);
+        const int r = t - band * 8 * p.ntiles;
+        nt = r / bh;
+        mt = m0 + (r - nt * bh);
+    } else {
+        mt = t / p.ntiles;
+        nt = t - mt * p.ntiles;
+    }
+}
+
 // ---- stream-K schedule (blocks of a *_sk_kernel) ----
 // Units are (tile, K step) pairs numbered tile * KT + k.  Block b < sk_dp
 // owns the units of data-parallel tile xcd_run(b, sk_dp); block sk_dp + i

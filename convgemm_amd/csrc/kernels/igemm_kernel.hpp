@@ -186,7 +186,8 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
     const int tile = skm ? lin : rest0 % tiles, rest = skm ? 0 : rest0 / tiles;
     const int split = rest % p.splits;
     const int cls = p.cls_order ? (int)((p.cls_order >> (4 * cslot)) & 15) : cslot;
-    const int nt = tile % p.ntiles, mt = tile / p.ntiles;
+    int mt, nt;
+    tile_mn(p, tile, mt, nt);
 
     const Ctx x = make_ctx(p, cls);
     if (!WG && mt * BM >= x.Mrows) return;  // uniform: DGRAD classes smaller than the grid

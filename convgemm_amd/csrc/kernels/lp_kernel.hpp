@@ -64,7 +64,8 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
 
     const int tiles = p.mtiles * p.ntiles;
     const int tile = lin % tiles, split = lin / tiles;
-    const int nt = tile % p.ntiles, mt = tile / p.ntiles;
+    int mt, nt;
+    tile_mn(p, tile, mt, nt);
     const Ctx x = make_ctx(p, 0);
 
     const __amdgpu_buffer_rsrc_t rA = make_rsrc(p.A, p.a_bytes);

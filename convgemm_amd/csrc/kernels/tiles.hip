@@ -1,6 +1,8 @@
 // tiles.hip -- the tile-configuration tables (tile_tables.hpp) behind
 // cg_internal.h's configuration queries, and the dispatch of a launch to the
-// instantiation slice (slice.hip) holding its kernels.
+// instantiation slice (slice.hip) holding its kernels.  A configuration is a
+// tile shape and a raster order (cg_tilecfg); the order is a run-time field
+// of the kernel parameters, so the slices hold the tile shapes only.
 //
 // Slices are referenced weakly: the build decides which (type, operation,
 // family) slices exist, and a slice that was not built resolves to null here
@@ -29,15 +31,26 @@ namespace {
 // [elem][op][family: 0 general, 1 pipelined]
 const slice_fn kSlices[3][CG_NUM_OPKINDS][2] = {CG_SLICE_ROW(f32), CG_SLICE_ROW(bf16), CG_SLICE_ROW(fp8)};
 
-template <typename T> const TileDef *table(int cfg) {
-    return cfg >= 0 && cfg < num_tiles<T>() ? &Tiles<T>::defs[cfg] : nullptr;
+int num_shapes(int elem) {
+    switch (elem) {
+    case CG_ELEM_F32: return num_tiles<float>();
+    case CG_ELEM_BF16: return num_tiles<uint16_t>();
+    case CG_ELEM_FP8: return num_tiles<uint8_t>();
+    }
+    return 0;
 }
 
-const TileDef *def_of(int elem, int cfg) {
+// Configuration cfg = tile shape base + raster order r * (number of shapes).
+const TileDef *def_of(int elem, int cfg, int *base = nullptr, int *raster = nullptr) {
+    const int ns = num_shapes(elem);
+    if (ns == 0 || cfg < 0 || cfg >= ns * CG_NUM_RASTERS) return nullptr;
+    const int b = cfg % ns;
+    if (base) *base = b;
+    if (raster) *raster = cfg / ns;
     switch (elem) {
-    case CG_ELEM_F32: return table<float>(cfg);
-    case CG_ELEM_BF16: return table<uint16_t>(cfg);
-    case CG_ELEM_FP8: return table<uint8_t>(cfg);
+    case CG_ELEM_F32: return &Tiles<float>::defs[b];
+    case CG_ELEM_BF16: return &Tiles<uint16_t>::defs[b];
+    case CG_ELEM_FP8: return &Tiles<uint8_t>::defs[b];
     }
     return nullptr;
 }
@@ -53,28 +66,26 @@ template <typename T> int supports_t(const TileDef &d, int op, int vec) {
 }
 
 int launch_any(int elem, int cfg, const cg_kparams *kp, const cg_group *g, int *occ, void *stream) {
-    const TileDef *d = def_of(elem, cfg);
+    int base, raster;
+    const TileDef *d = def_of(elem, cfg, &base, &raster);
     if (!d || kp->op < 0 || kp->op >= CG_NUM_OPKINDS) return (int)hipErrorInvalidValue;
+    if (!occ && !g && kp->raster != raster) return (int)hipErrorInvalidValue;  // tiles filled for another order
     const slice_fn f = kSlices[elem][kp->op][d->fam == FAM_GEN ? 0 : 1];
     if (!f) return (int)hipErrorInvalidValue;
-    return f(cfg, kp, g, occ, stream);
+    return f(base, kp, g, occ, stream);
 }
 
 }  // namespace
 
-extern "C" int cg_num_tilecfgs(int elem) {
-    switch (elem) {
-    case CG_ELEM_F32: return num_tiles<float>();
-    case CG_ELEM_BF16: return num_tiles<uint16_t>();
-    case CG_ELEM_FP8: return num_tiles<uint8_t>();
-    }
-    return 0;
-}
+extern "C" int cg_num_tilecfgs(int elem) { return num_shapes(elem) * CG_NUM_RASTERS; }
+
+extern "C" int cg_num_tile_shapes(int elem) { return num_shapes(elem); }
 
 extern "C" int cg_get_tilecfg(int elem, int cfg, cg_tilecfg *out) {
-    const TileDef *d = def_of(elem, cfg);
+    int raster;
+    const TileDef *d = def_of(elem, cfg, nullptr, &raster);
     if (!d) return -1;
-    *out = cg_tilecfg{d->BM, d->BN, d->BK, d->WM, d->WN, d->NS};
+    *out = cg_tilecfg{d->BM, d->BN, d->BK, d->WM, d->WN, d->NS, raster};
     return 0;
 }
 

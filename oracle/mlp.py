@@ -4,11 +4,12 @@ TEST INFRASTRUCTURE ONLY: checks the device forward/backward kernels of
 convgemm_amd/csrc/kernels/mlp.hip.  Network: x (F) -> relu(x W1 + b1) (H)
 -> relu(h1 W2 + b2) (H) -> h2 W3 + b3 (1); loss = mean((y - t)^2).
 Parameters are one flat vector [W1 (FxH), b1, W2 (HxH), b2, W3 (Hx1), b3],
-row-major.  Pinned in tests/test_oracle.py by central finite differences.
+row-major.  Pinned in tests/test_oracle.py by central finite differences and
+against torch's float64 linear layers and autograd.
 """
 import numpy as np
 
-F, H = 30, 64
+F, H = 32, 64
 NP = F * H + H + H * H + H + H + 1
 
 

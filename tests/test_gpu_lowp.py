@@ -146,7 +146,7 @@ def test_fp8_rejects_backward_and_missing_scales():
 @pytest.mark.parametrize("dtype", ["bf16", "fp8"])
 def test_every_lowp_configuration(dtype):
     elem = cg.BF16 if dtype == "bf16" else cg.FP8_E4M3
-    ncfg = len(cg.tile_configs(elem))
+    ncfg = cg.num_tile_shapes(elem)
     s = Shape(2, 11, 10, 64, 80, 3, 3, 1, 1, 1, 1, 1, 1)
     rng = np.random.default_rng(5)
     for layout in (NHWC, NCHW):
@@ -176,7 +176,7 @@ def test_every_bf16_configuration_backward(splits):
     offers them: 16-byte channel groups (C, kn multiples of 8; the weight
     gradient's transposing loader in NHWC) and element loads (C = 12), with
     split-K and stream-K (splits 0, weight gradient)."""
-    ncfg = len(cg.tile_configs(cg.BF16))
+    ncfg = cg.num_tile_shapes(cg.BF16)
     cases = [(Shape(2, 11, 10, 64, 80, 3, 3, 1, 1, 1, 1, 1, 1), True),
              (Shape(2, 9, 8, 12, 24, 3, 3, 1, 1, 2, 1, 1, 1), False),
              (Shape(3, 7, 7, 40, 16, 1, 1, 0, 0, 1, 1, 1, 1), True)]
@@ -226,7 +226,7 @@ LP_SHAPES = [
 @pytest.mark.parametrize("si", range(len(LP_SHAPES)))
 def test_pipelined_forward_configurations(dtype, si):
     elem = {"f32": cg.F32, "bf16": cg.BF16, "fp8": cg.FP8_E4M3}[dtype]
-    lp = [i for i, t in enumerate(cg.tile_configs(elem)) if t[5] > 0 and cg.tile_supports(elem, i, cg.FWD, NHWC)]
+    lp = [i for i, t in enumerate(cg.tile_configs(elem)) if t[5] > 0 and t[6] == 0 and cg.tile_supports(elem, i, cg.FWD, NHWC)]
     assert lp, "no pipelined configurations"
     s = LP_SHAPES[si]
     if dtype == "fp8" and s.c % 16:
@@ -269,7 +269,7 @@ def test_pipelined_forward_nchw_fp32(si):
     K-major LDS tile) on every tail, padding, stride and dilation case, with
     the fused epilogue and split-K, against the FP64 oracle."""
     lpn = [i for i, t in enumerate(cg.tile_configs(cg.F32))
-           if t[5] > 0 and cg.tile_supports(cg.F32, i, cg.FWD, NCHW)]
+           if t[5] > 0 and t[6] == 0 and cg.tile_supports(cg.F32, i, cg.FWD, NCHW)]
     assert lpn, "no pipelined NCHW configurations"
     s = LP_SHAPES[si]
     rng = np.random.default_rng(70 + si)
@@ -311,7 +311,7 @@ def test_pipelined_stream_k(dtype, layout):
     shared between workgroups are summed by the fix-up kernel; forward with
     bias + ReLU against the FP64 oracle."""
     elem = {"f32": cg.F32, "bf16": cg.BF16, "fp8": cg.FP8_E4M3}[dtype]
-    cfgs = [i for i, t in enumerate(cg.tile_configs(elem)) if t[5] > 0 and cg.tile_supports(elem, i, cg.FWD, layout)]
+    cfgs = [i for i, t in enumerate(cg.tile_configs(elem)) if t[5] > 0 and t[6] == 0 and cg.tile_supports(elem, i, cg.FWD, layout)]
     assert cfgs
     tol = {"f32": 1e-5, "bf16": 2e-5, "fp8": FP8_TOL}[dtype]
     for si, s in enumerate(SK_LP_SHAPES):

@@ -324,3 +324,30 @@ def test_mlp_oracle_gradient_matches_finite_differences():
         e[i] = h
         fd = (mlp.loss(p + e, X, T) - mlp.loss(p - e, X, T)) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, fd, g[i])
+
+
+def test_mlp_oracle_matches_torch_autograd():
+    """The MLP oracle's forward pass and loss gradient against an independent
+    implementation: torch's float64 linear layers and autograd on the same
+    flat parameter layout [W1 (FxH), b1, W2 (HxH), b2, W3 (Hx1), b3]."""
+    import torch
+    from oracle import mlp
+    rng = np.random.default_rng(12)
+    p = rng.standard_normal(mlp.NP) * 0.2
+    X = rng.standard_normal((23, mlp.F))
+    T = rng.standard_normal(23)
+    pt = torch.tensor(p, dtype=torch.float64, requires_grad=True)
+    F, H = mlp.F, mlp.H
+    o = [0, F * H, F * H + H, F * H + H + H * H, F * H + 2 * H + H * H, F * H + 3 * H + H * H, mlp.NP]
+    W1, b1, W2, b2, W3, b3 = (pt[o[i]:o[i + 1]] for i in range(6))
+    Xt = torch.tensor(X, dtype=torch.float64)
+    h1 = torch.relu(torch.nn.functional.linear(Xt, W1.reshape(F, H).t(), b1))
+    h2 = torch.relu(torch.nn.functional.linear(h1, W2.reshape(H, H).t(), b2))
+    y = torch.nn.functional.linear(h2, W3.reshape(H, 1).t(), b3)[:, 0]
+    loss = torch.mean((y - torch.tensor(T, dtype=torch.float64)) ** 2)
+    loss.backward()
+    y_o, _ = mlp.forward(p, X)
+    L_o, g_o = mlp.loss_grad(p, X, T)
+    np.testing.assert_allclose(y_o, y.detach().numpy(), rtol=1e-12, atol=1e-12)
+    assert abs(L_o - loss.item()) <= 1e-12 * max(1.0, abs(L_o))
+    np.testing.assert_allclose(g_o, pt.grad.numpy(), rtol=1e-10, atol=1e-12)
