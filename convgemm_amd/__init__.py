@@ -209,7 +209,7 @@ def lib() -> ctypes.CDLL:
     L.cg_split_factors.restype = ci
     L.cg_num_tilecfgs.argtypes = [ci]
     L.cg_num_tilecfgs.restype = ci
-    L.cg_get_tilecfg.argtypes = [ci, ci, ctypes.POINTER(ci * 7)]
+    L.cg_get_tilecfg.argtypes = [ci, ci, ctypes.POINTER(ci * 8)]
     L.cg_num_tile_shapes.argtypes = [ci]
     L.cg_num_tile_shapes.restype = ci
     L.cg_get_tilecfg.restype = ci
@@ -686,13 +686,15 @@ def weights_nchw_to_nhwc(src, dst, kn, c, kh, kw, esize=4, stream=None):
 
 def tile_configs(dtype=F32):
     """Configurations of an operand type: list of (BM, BN, BK, WM, WN, pipelined stages or 0,
-    raster order).  Configuration base + r * num_tile_shapes(dtype) is tile shape `base`
-    walked in raster order r (0: the N panels of an M panel adjacent, 1: the M panels of an
-    N panel adjacent, 2: bands of 8 M panels)."""
+    raster order, halo slots or 0).  Configuration base + r * num_tile_shapes(dtype) is tile
+    shape `base` walked in raster order r (0: the N panels of an M panel adjacent, 1: the M
+    panels of an N panel adjacent, 2: bands of 8 M panels).  A configuration with halo slots
+    is the halo-tile forward kernel: its M tiles hold the pixels of one image, and it serves
+    a problem only when the input rows those pixels read fit the slots."""
     L = lib()
     out = []
     for c in range(L.cg_num_tilecfgs(int(dtype))):
-        v = (ctypes.c_int * 7)()
+        v = (ctypes.c_int * 8)()
         if L.cg_get_tilecfg(int(dtype), c, ctypes.byref(v)) == 0:
             out.append(tuple(v))
     return out

@@ -78,6 +78,8 @@ typedef struct {
     int WM, WN;      /* waves along M and N (WM*WN = 4, or 8 for some pipelined tiles) */
     int lp;          /* > 0: a pipelined forward kernel (NHWC; NCHW for FP32) with lp LDS stages */
     int raster;      /* tile order (cg_kparams.raster) */
+    int halo;        /* > 0: the halo-tile forward NHWC kernel with this many 128-byte halo slots; its M
+                        tiles are per image (mtiles = B * ceil(Ho*Wo / BM)) */
 } cg_tilecfg;
 #define CG_NUM_RASTERS 3
 

@@ -20,6 +20,10 @@ void cg_set_plan_hook(cg_plan_hook_t hook, void *ctx);
 /* Force every call to use (cfg, splits); cfg < 0 clears it. Tuning hook. */
 void cg_force_config(int cfg, int splits);
 void cg_fill_tiles(cg_kparams *kp, const cg_tilecfg *t, int splits);
+/* Does the problem's largest halo fit a halo-tile configuration (1 for the others)? */
+int cg_halo_fits(const cg_kparams *kp, const cg_tilecfg *t);
+/* Configuration cfg serves the problem (operation, loads, halo). */
+int cg_plan_usable(int elem, int cfg, const cg_kparams *kp);
 /* The analytic ("no learning") planner. */
 void cg_default_plan(int elem, const cg_kparams *base, int *cfg, int *splits);
 size_t cg_partial_bytes(const cg_kparams *kp);

@@ -341,6 +341,14 @@ void cg_fill_tiles(cg_kparams *kp, const cg_tilecfg *t, int splits) {
         kp->N = kp->KN;
         kp->kchunks = cdiv(kp->C, t->BK);
         kp->KT = kp->taps * kp->kchunks;
+        if (t->halo) { /* halo tiles: BM pixels of one image; split-K divides the channel chunks */
+            kp->mtiles = kp->B * cdiv((long long)kp->Ho * kp->Wo, t->BM);
+            kp->ntiles = cdiv(kp->N, t->BN);
+            if (splits < 1) splits = 1;
+            if (splits > kp->kchunks) splits = kp->kchunks > 0 ? kp->kchunks : 1;
+            kp->splits = splits;
+            return;
+        }
     } else if (op == CG_DGRAD_NHWC || op == CG_DGRAD_NCHW) {
         kp->ncls_h = kp->vs;
         kp->ncls_w = kp->hs;
@@ -366,6 +374,35 @@ void cg_fill_tiles(cg_kparams *kp, const cg_tilecfg *t, int splits) {
     kp->splits = splits;
 }
 
+/* Does the largest halo of the problem fit the configuration?  The halo of an
+ * M tile (BM consecutive output pixels of one image, rows r_first..r_last) is
+ * ((r_last - r_first) * vs + (KH - 1) * vd + 1) input rows of
+ * (Wo - 1) * hs + (KW - 1) * hd + 1 slots (ht_kernel.hpp).  1x1 kernels gain
+ * nothing from a halo and are not served. */
+int cg_halo_fits(const cg_kparams *kp, const cg_tilecfg *t) {
+    if (!t->halo) return 1;
+    if (kp->op != CG_FWD_NHWC || kp->taps <= 1) return 0;
+    const long long howo = (long long)kp->Ho * kp->Wo;
+    if (howo <= 0) return 0;
+    const long long hc = (long long)(kp->Wo - 1) * kp->hs + (long long)(kp->KW - 1) * kp->hd + 1;
+    const long long tpi = (howo + t->BM - 1) / t->BM;
+    for (long long j = 0; j < tpi; j++) {
+        const long long q0 = j * t->BM, q1 = (q0 + t->BM < howo ? q0 + t->BM : howo) - 1;
+        const long long rows = q1 / kp->Wo - q0 / kp->Wo;
+        const long long hr = rows * kp->vs + (long long)(kp->KH - 1) * kp->vd + 1;
+        if (hr * hc > t->halo) return 0;
+    }
+    return 1;
+}
+
+/* A plan the kernels can run for this problem: the configuration serves the
+ * operation and operand layout, and a halo tile's halo fits. */
+int cg_plan_usable(int elem, int cfg, const cg_kparams *kp) {
+    cg_tilecfg t;
+    if (cfg < 0 || cg_get_tilecfg(elem, cfg, &t) || !cg_tilecfg_supports(elem, cfg, kp->op, kp->vec)) return 0;
+    return cg_halo_fits(kp, &t);
+}
+
 size_t cg_partial_bytes(const cg_kparams *kp) {
     if (kp->splits <= 1) return 0;
     return (size_t)kp->splits * kp->ncls_h * kp->ncls_w * (size_t)kp->M * kp->N * sizeof(float);
@@ -383,7 +420,7 @@ void cg_default_plan(int elem, const cg_kparams *base, int *cfg_out, int *splits
     int n = cg_num_tile_shapes(elem); /* raster order 0: the N panels of an M panel adjacent */
     for (int cfg = 0; cfg < n; cfg++) {
         cg_tilecfg t;
-        if (cg_get_tilecfg(elem, cfg, &t) || !cg_tilecfg_supports(elem, cfg, op, base->vec)) continue;
+        if (cg_get_tilecfg(elem, cfg, &t) || !cg_tilecfg_supports(elem, cfg, op, base->vec) || t.halo) continue;
         cg_kparams kp = *base;
         cg_fill_tiles(&kp, &t, 1);
         long long tiles = (long long)kp.mtiles * kp.ntiles * kp.ncls_h * kp.ncls_w;
@@ -544,13 +581,13 @@ int cg_planning_in_capture(void) { return t_in_capture; }
 static int choose_plan(int elem, const cg_kparams *kp, int *cfg, int *splits) {
     *cfg = -1;
     *splits = 1;
-    if (g_force_cfg >= 0 && cg_tilecfg_supports(elem, g_force_cfg, kp->op, kp->vec)) {
+    if (g_force_cfg >= 0 && cg_plan_usable(elem, g_force_cfg, kp)) {
         *cfg = g_force_cfg;
         *splits = g_force_splits >= 0 ? g_force_splits : 1; /* 0: stream-K */
     } else if (g_plan_hook) {
         g_plan_hook(g_plan_hook_ctx, elem, kp, cfg, splits);
     }
-    if (*cfg < 0 || !cg_tilecfg_supports(elem, *cfg, kp->op, kp->vec)) cg_default_plan(elem, kp, cfg, splits);
+    if (!cg_plan_usable(elem, *cfg, kp)) cg_default_plan(elem, kp, cfg, splits);
     return *cfg;
 }
 

@@ -1,7 +1,7 @@
 /*
  * costmodel.c -- learned cost model for tile-configuration selection.
  *
- * An MLP  F=32 features -> 64 -> 64 -> 1  (ReLU) predicts the standardised
+ * An MLP  F=33 features -> 64 -> 64 -> 1  (ReLU) predicts the standardised
  * log time of one (problem, candidate) pair.  Features (cg_features), log2
  * where a size:
  *   0-5   operation one-hot (forward / weight gradient / backward data x NHWC, NCHW)
@@ -13,7 +13,8 @@
  *   24    waves of workgroups over 256 CUs at the tile's LDS occupancy
  *   25    computed FLOPs (with tile padding)
  *   26    pipelined kernel;  27  stream-K;  28  waves per workgroup;  29  LDS stages
- *   30-31 raster order one-hot (order 0, the N panels of an M panel adjacent, is neither)  It is trained on every timing
+ *   30-31 raster order one-hot (order 0, the N panels of an M panel adjacent, is neither)
+ *   32    the halo-tile kernel  It is trained on every timing
  * the autotuner recorded (tune.c) by the device kernels of mlp.hip --
  * forward, backward and Adam on the GPU -- and evaluated by the same forward
  * kernels to rank the candidates of a problem that was never tuned.  The
@@ -34,8 +35,9 @@
 #include "../cg_internal.h"
 #include "cg_host.h"
 
-#define CM_F 32
-#define CM_F_V1 30 /* models saved before the raster-order features load with those features at 0 */
+#define CM_F 33
+#define CM_F_V1 30 /* models saved with fewer features (before the raster-order and halo
+                      * features) load with the later features' weights at 0 */
 #define CM_H 64
 #define CM_NP (CM_F * CM_H + CM_H + CM_H * CM_H + CM_H + CM_H + 1)
 #define OFF_W1 0
@@ -113,6 +115,7 @@ int cg_features(int elem, const cg_kparams *kp0, int cfg, int splits, float *f) 
     f[29] = (float)t.lp;                           /* LDS stages of the pipelined kernels */
     f[30] = t.raster == 1 ? 1.0f : 0.0f;           /* raster order: M panels adjacent */
     f[31] = t.raster == 2 ? 1.0f : 0.0f;           /*               bands of 8 M panels */
+    f[32] = t.halo ? 1.0f : 0.0f;                  /* the halo-tile kernel */
     if (splits == 0) { /* stream-K: data-parallel tiles + stream-K workgroups, no split partials */
         int dp;
         const int g = cg_sk_split(elem, cfg, &kp, &dp);
@@ -494,10 +497,10 @@ convgemm_status_t convgemm_costmodel_load(const char *path) {
     char mg[8];
     int dims[3];
     int ok = fread(mg, 1, 8, f) == 8 && !memcmp(mg, MAGIC, 8) && fread(dims, sizeof(int), 3, f) == 3;
-    /* this build's feature count, or the earlier one: its first-layer rows for the later
+    /* this build's feature count, or an earlier one: its first-layer rows for the later
      * features are zero (inputs the network ignores) */
     const int F = ok ? dims[0] : 0;
-    ok = ok && (F == CM_F || F == CM_F_V1) && dims[1] == CM_H &&
+    ok = ok && F >= CM_F_V1 && F <= CM_F && dims[1] == CM_H &&
          dims[2] == F * CM_H + CM_H + CM_H * CM_H + CM_H + CM_H + 1;
     for (int j = 0; j < CM_F; j++) {
         m.mu[j] = 0.0f;

@@ -159,7 +159,7 @@ int cg_candidates(int elem, const cg_kparams *base, int *cfgs, int *splits, int 
     int n = 0, ncfg = cg_num_tilecfgs(elem);
     for (int c = 0; c < ncfg; c++) {
         cg_tilecfg t;
-        if (cg_get_tilecfg(elem, c, &t) || !cg_tilecfg_supports(elem, c, base->op, base->vec)) continue;
+        if (cg_get_tilecfg(elem, c, &t) || !cg_plan_usable(elem, c, base)) continue;
         cg_kparams kp = *base;
         cg_fill_tiles(&kp, &t, 1);
         /* a raster order that walks the tiles as order 0 does (one M or one N
@@ -170,6 +170,7 @@ int cg_candidates(int elem, const cg_kparams *base, int *cfgs, int *splits, int 
         for (unsigned si = 0; si < sizeof(S) / sizeof(S[0]); si++) {
             int s = S[si];
             if (s > 1 && kp.KT / s < 2) break;
+            if (t.halo && s > kp.kchunks) break; /* halo tiles split whole channel chunks */
             cg_kparams k2 = kp;
             cg_fill_tiles(&k2, &t, s);
             if (cg_partial_bytes(&k2) > ((size_t)CG_PACK_BYTES)) break;

@@ -9,6 +9,7 @@
 #include "igemm_kernel.hpp"
 #include "lp_kernel.hpp"
 #include "lp_nchw_kernel.hpp"
+#include "ht_kernel.hpp"
 #include "tile_tables.hpp"
 
 using namespace cg;
@@ -28,6 +29,8 @@ int launch_slice(int cfg, const cg_kparams *kp, const cg_group *g, int *occ, hip
                 return launch_lp<T, d.BM, d.BN, d.WM, d.WN, d.NS>(kp, g, occ, st);
             else if constexpr (FAM == 1 && d.fam == FAM_LPN && OP == CG_FWD_NCHW && std::is_same<T, float>::value)
                 return launch_lpn<d.BM, d.BN, d.WM, d.WN, d.NS>(kp, g, occ, st);
+            else if constexpr (FAM == 1 && d.fam == FAM_HT && OP == CG_FWD_NHWC)
+                return launch_ht<T, d.BM, d.BN, d.WM, d.WN, d.NS, d.HS>(kp, g, occ, st);
             else
                 return (int)hipErrorInvalidValue;
         }

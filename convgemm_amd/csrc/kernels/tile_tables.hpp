@@ -8,9 +8,11 @@
 //   NS       LDS stages of a pipelined kernel (0: the general kernel)
 //   fam      kernel family: GEN = igemm_tile (every operation), LP = the
 //            pipelined forward NHWC kernel, LPN = the pipelined forward NCHW
-//            kernel (FP32)
+//            kernel (FP32), HT = the halo-tile forward NHWC kernel
+//            (ht_kernel.hpp; kernels larger than 1x1 whose halo fits HS)
 //   nv       the general kernel also has element-load (non-16-byte)
 //            instantiations of this tile
+//   HS       halo slots (128 bytes each) of an HT tile
 #pragma once
 #include <stdint.h>
 
@@ -18,12 +20,13 @@
 
 namespace cg {
 
-enum TileFam { FAM_GEN = 0, FAM_LP = 1, FAM_LPN = 2 };
+enum TileFam { FAM_GEN = 0, FAM_LP = 1, FAM_LPN = 2, FAM_HT = 3 };
 
 struct TileDef {
     int BM, BN, BK, WM, WN, NS;
     int fam;
     bool nv;
+    int HS = 0;
 };
 
 template <typename T> struct Tiles;
@@ -62,6 +65,10 @@ template <> struct Tiles<float> {
         {128, 128, 32, 2, 4, 3, FAM_LP, false},   // 29 8 waves
         {256, 64, 32, 4, 2, 3, FAM_LP, false},    // 30 8 waves
         {64, 64, 64, 2, 2, 0, FAM_GEN, false},    // 31 long K step (weight gradient: fewer barriers per MFMA)
+        {128, 64, 32, 2, 2, 3, FAM_HT, false, 384},   // 32 halo tile
+        {256, 64, 32, 4, 1, 2, FAM_HT, false, 512},   // 33 halo tile
+        {128, 128, 32, 2, 2, 2, FAM_HT, false, 384},  // 34 halo tile
+        {64, 64, 32, 2, 2, 3, FAM_HT, false, 256},    // 35 halo tile
     };
 };
 
@@ -98,6 +105,10 @@ template <> struct Tiles<uint16_t> {
         {256, 64, 64, 4, 2, 3, FAM_LP, false},    // 28 8 waves
         {128, 128, 64, 2, 4, 4, FAM_LP, false},   // 29 8 waves
         {64, 64, 128, 2, 2, 0, FAM_GEN, false},   // 30 long K step (weight gradient: fewer barriers per MFMA)
+        {128, 64, 64, 2, 2, 3, FAM_HT, false, 384},   // 31 halo tile
+        {256, 64, 64, 4, 1, 2, FAM_HT, false, 512},   // 32 halo tile
+        {128, 128, 64, 2, 2, 2, FAM_HT, false, 384},  // 33 halo tile
+        {64, 64, 64, 2, 2, 3, FAM_HT, false, 256},    // 34 halo tile
     };
 };
 
@@ -132,6 +143,10 @@ template <> struct Tiles<uint8_t> {
         {512, 64, 128, 8, 1, 2, FAM_LP, false},   // 26 8 waves
         {256, 64, 128, 4, 2, 3, FAM_LP, false},   // 27 8 waves
         {128, 128, 128, 2, 4, 4, FAM_LP, false},  // 28 8 waves
+        {128, 64, 128, 2, 2, 3, FAM_HT, false, 384},   // 29 halo tile
+        {256, 64, 128, 4, 1, 2, FAM_HT, false, 512},   // 30 halo tile
+        {128, 128, 128, 2, 2, 2, FAM_HT, false, 384},  // 31 halo tile
+        {64, 64, 128, 2, 2, 3, FAM_HT, false, 256},    // 32 halo tile
     };
 };
 

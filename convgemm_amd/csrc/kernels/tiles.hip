@@ -56,7 +56,7 @@ const TileDef *def_of(int elem, int cfg, int *base = nullptr, int *raster = null
 }
 
 template <typename T> int supports_t(const TileDef &d, int op, int vec) {
-    if (d.fam == FAM_LP) return op == CG_FWD_NHWC && vec;
+    if (d.fam == FAM_LP || d.fam == FAM_HT) return op == CG_FWD_NHWC && vec;
     if (d.fam == FAM_LPN) return op == CG_FWD_NCHW && vec;
     if (!op_enabled<T>(op)) return 0;
     if (!vec && vec_relevant<T>(op) && !d.nv) return 0;
@@ -85,7 +85,7 @@ extern "C" int cg_get_tilecfg(int elem, int cfg, cg_tilecfg *out) {
     int raster;
     const TileDef *d = def_of(elem, cfg, nullptr, &raster);
     if (!d) return -1;
-    *out = cg_tilecfg{d->BM, d->BN, d->BK, d->WM, d->WN, d->NS, raster};
+    *out = cg_tilecfg{d->BM, d->BN, d->BK, d->WM, d->WN, d->NS, raster, d->HS};
     return 0;
 }
 

@@ -9,7 +9,7 @@ against torch's float64 linear layers and autograd.
 """
 import numpy as np
 
-F, H = 32, 64
+F, H = 33, 64
 NP = F * H + H + H * H + H + H + 1
 
 

@@ -204,7 +204,7 @@ def pipelined_cases(dtype, layout, n, seed):
 def test_pipelined_groups(dtype, layout, flags):
     """Grouped (and persistent) launches of the pipelined kernels: every
     pipelined configuration forced in turn, with and without split-K."""
-    cfgs = [i for i, t in enumerate(cg.tile_configs(dtype)) if t[5] > 0 and t[6] == 0 and cg.tile_supports(dtype, i, cg.FWD, layout)]
+    cfgs = [i for i, t in enumerate(cg.tile_configs(dtype)) if t[5] > 0 and t[6] == 0 and t[7] == 0 and cg.tile_supports(dtype, i, cg.FWD, layout)]
     assert cfgs
     for cfg in cfgs:
         for splits in (1, 2):

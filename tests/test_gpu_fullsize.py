@@ -103,7 +103,7 @@ def test_fullsize_pipelined_forward(name, layout):
     remainder): 256 sampled outputs against the FP64 oracle."""
     s = LAYERS[name]
     cfgs = [i for i, t in enumerate(cg.tile_configs(cg.F32)[:cg.num_tile_shapes(cg.F32)])
-            if t[5] > 0 and cg.tile_supports(cg.F32, i, cg.FWD, layout)]
+            if t[5] > 0 and t[7] == 0 and cg.tile_supports(cg.F32, i, cg.FWD, layout)]
     assert cfgs
     g = torch.Generator(device="cuda").manual_seed(sum(map(ord, name)) * 3 + layout)
     X = torch.randn(s.x_shape(layout), device="cuda", generator=g)

@@ -226,7 +226,7 @@ LP_SHAPES = [
 @pytest.mark.parametrize("si", range(len(LP_SHAPES)))
 def test_pipelined_forward_configurations(dtype, si):
     elem = {"f32": cg.F32, "bf16": cg.BF16, "fp8": cg.FP8_E4M3}[dtype]
-    lp = [i for i, t in enumerate(cg.tile_configs(elem)) if t[5] > 0 and t[6] == 0 and cg.tile_supports(elem, i, cg.FWD, NHWC)]
+    lp = [i for i, t in enumerate(cg.tile_configs(elem)) if t[5] > 0 and t[6] == 0 and t[7] == 0 and cg.tile_supports(elem, i, cg.FWD, NHWC)]
     assert lp, "no pipelined configurations"
     s = LP_SHAPES[si]
     if dtype == "fp8" and s.c % 16:
@@ -311,7 +311,7 @@ def test_pipelined_stream_k(dtype, layout):
     shared between workgroups are summed by the fix-up kernel; forward with
     bias + ReLU against the FP64 oracle."""
     elem = {"f32": cg.F32, "bf16": cg.BF16, "fp8": cg.FP8_E4M3}[dtype]
-    cfgs = [i for i, t in enumerate(cg.tile_configs(elem)) if t[5] > 0 and t[6] == 0 and cg.tile_supports(elem, i, cg.FWD, layout)]
+    cfgs = [i for i, t in enumerate(cg.tile_configs(elem)) if t[5] > 0 and t[6] == 0 and t[7] == 0 and cg.tile_supports(elem, i, cg.FWD, layout)]
     assert cfgs
     tol = {"f32": 1e-5, "bf16": 2e-5, "fp8": FP8_TOL}[dtype]
     for si, s in enumerate(SK_LP_SHAPES):
