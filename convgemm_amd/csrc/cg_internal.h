@@ -67,6 +67,13 @@ typedef struct {
                         N panel adjacent, 2 bands of 8 M panels (M fastest within a band) */
     int sk_dp;       /* with sk: tiles 0 .. sk_dp-1 run one per workgroup (blocks 0 .. sk_dp-1), the
                         sk workgroups after them share the K steps of tiles sk_dp .. mtiles*ntiles-1 */
+    int hrows, hcblk; /* halo tiles: whole output rows per M tile (Wo <= BM), or M tiles per output
+                         row (Wo > BM); the other is 1 */
+    int htpi, hpitch; /* halo tiles: M tiles per image, and slots per halo row (that of a full tile) */
+    /* Division by the problem's divisors as a multiply and shift (cg_fill_tiles):
+     * n / d == (n * m) >> s for 0 <= n < 2^31; d = Ho*Wo, Wo, C, hpitch. */
+    unsigned fdm_howo, fdm_wo, fdm_c, fdm_hp;
+    int fds_howo, fds_wo, fds_c, fds_hp;
 } cg_kparams;
 
 /* Tile configurations of the implicit-GEMM kernels: a tile shape and kernel
@@ -79,7 +86,7 @@ typedef struct {
     int lp;          /* > 0: a pipelined forward kernel (NHWC; NCHW for FP32) with lp LDS stages */
     int raster;      /* tile order (cg_kparams.raster) */
     int halo;        /* > 0: the halo-tile forward NHWC kernel with this many 128-byte halo slots; its M
-                        tiles are per image (mtiles = B * ceil(Ho*Wo / BM)) */
+                        tiles are whole output rows of one image, or pieces of one row (hrows, hcblk) */
 } cg_tilecfg;
 #define CG_NUM_RASTERS 3
 

@@ -330,9 +330,36 @@ static unsigned long long dgrad_class_order(const cg_kparams *kp) {
 }
 
 /* Fill the tile-space fields of kp for configuration t. */
+/* Halo-tile M tiling: whole output rows per tile when a row fits the BM tile
+ * rows, else BM-column pieces of a row. */
+static void halo_tiling(const cg_kparams *kp, int BM, int *rows, int *cblk) {
+    if (kp->Wo <= BM) {
+        *rows = kp->Wo > 0 ? BM / kp->Wo : 1;
+        *cblk = 1;
+    } else {
+        *rows = 1;
+        *cblk = cdiv(kp->Wo, BM);
+    }
+}
+
+/* Multiply-and-shift constants of n / d for 0 <= n < 2^31 (the kernels'
+ * FastDiv, igemm_common.hpp): s = 31 + ceil(log2 d), m = ceil(2^s / d). */
+static void fastdiv_magic(long long d, unsigned *m, int *s) {
+    if (d < 1) d = 1;
+    int l = 0;
+    while ((1ll << l) < d) l++;
+    *s = 31 + l;
+    *m = (unsigned)(((1ull << *s) + (unsigned long long)d - 1) / (unsigned long long)d);
+}
+
 void cg_fill_tiles(cg_kparams *kp, const cg_tilecfg *t, int splits) {
     int op = kp->op;
     kp->raster = t->raster;
+    fastdiv_magic((long long)kp->Ho * kp->Wo, &kp->fdm_howo, &kp->fds_howo);
+    fastdiv_magic(kp->Wo, &kp->fdm_wo, &kp->fds_wo);
+    fastdiv_magic(kp->C, &kp->fdm_c, &kp->fds_c);
+    kp->hrows = kp->hcblk = kp->htpi = kp->hpitch = 0;
+    fastdiv_magic(1, &kp->fdm_hp, &kp->fds_hp);
     kp->ncls_h = 1;
     kp->ncls_w = 1;
     kp->cls_order = 0;
@@ -341,8 +368,13 @@ void cg_fill_tiles(cg_kparams *kp, const cg_tilecfg *t, int splits) {
         kp->N = kp->KN;
         kp->kchunks = cdiv(kp->C, t->BK);
         kp->KT = kp->taps * kp->kchunks;
-        if (t->halo) { /* halo tiles: BM pixels of one image; split-K divides the channel chunks */
-            kp->mtiles = kp->B * cdiv((long long)kp->Ho * kp->Wo, t->BM);
+        if (t->halo) { /* halo tiles: whole output rows, or pieces of one row; split-K divides the
+                          channel chunks */
+            halo_tiling(kp, t->BM, &kp->hrows, &kp->hcblk);
+            kp->htpi = cdiv(kp->Ho, kp->hrows) * kp->hcblk;
+            kp->hpitch = ((kp->hcblk == 1 ? kp->Wo : t->BM) - 1) * kp->hs + (kp->KW - 1) * kp->hd + 1;
+            fastdiv_magic(kp->hpitch, &kp->fdm_hp, &kp->fds_hp);
+            kp->mtiles = kp->B * kp->htpi;
             kp->ntiles = cdiv(kp->N, t->BN);
             if (splits < 1) splits = 1;
             if (splits > kp->kchunks) splits = kp->kchunks > 0 ? kp->kchunks : 1;
@@ -374,25 +406,21 @@ void cg_fill_tiles(cg_kparams *kp, const cg_tilecfg *t, int splits) {
     kp->splits = splits;
 }
 
-/* Does the largest halo of the problem fit the configuration?  The halo of an
- * M tile (BM consecutive output pixels of one image, rows r_first..r_last) is
- * ((r_last - r_first) * vs + (KH - 1) * vd + 1) input rows of
- * (Wo - 1) * hs + (KW - 1) * hd + 1 slots (ht_kernel.hpp).  1x1 kernels gain
- * nothing from a halo and are not served. */
+/* Does the largest halo of the problem fit the configuration?  A halo tile
+ * covers `rows` whole output rows, or `cols` <= BM columns of one row, and its
+ * halo is ((rows - 1) * vs + (KH - 1) * vd + 1) input rows of
+ * ((cols - 1) * hs + (KW - 1) * hd + 1) slots (ht_kernel.hpp); full tiles are
+ * the largest.  1x1 kernels gain nothing from a halo and are not served. */
 int cg_halo_fits(const cg_kparams *kp, const cg_tilecfg *t) {
     if (!t->halo) return 1;
-    if (kp->op != CG_FWD_NHWC || kp->taps <= 1) return 0;
-    const long long howo = (long long)kp->Ho * kp->Wo;
-    if (howo <= 0) return 0;
-    const long long hc = (long long)(kp->Wo - 1) * kp->hs + (long long)(kp->KW - 1) * kp->hd + 1;
-    const long long tpi = (howo + t->BM - 1) / t->BM;
-    for (long long j = 0; j < tpi; j++) {
-        const long long q0 = j * t->BM, q1 = (q0 + t->BM < howo ? q0 + t->BM : howo) - 1;
-        const long long rows = q1 / kp->Wo - q0 / kp->Wo;
-        const long long hr = rows * kp->vs + (long long)(kp->KH - 1) * kp->vd + 1;
-        if (hr * hc > t->halo) return 0;
-    }
-    return 1;
+    if (kp->op != CG_FWD_NHWC || kp->taps <= 1 || kp->Ho <= 0 || kp->Wo <= 0) return 0;
+    int rpt, cblk;
+    halo_tiling(kp, t->BM, &rpt, &cblk);
+    const long long rows = rpt < kp->Ho ? rpt : kp->Ho;
+    const long long cols = cblk == 1 ? kp->Wo : t->BM;
+    const long long hr = (rows - 1) * kp->vs + (long long)(kp->KH - 1) * kp->vd + 1;
+    const long long hc = (cols - 1) * kp->hs + (long long)(kp->KW - 1) * kp->hd + 1;
+    return hr * hc <= t->halo;
 }
 
 /* A plan the kernels can run for this problem: the configuration serves the

@@ -4,12 +4,14 @@
 // 1x1.  The pipelined kernel (lp_kernel.hpp) gathers the patch rows of every
 // tap from HBM/L2 again: a 3x3 stride-1 layer loads each input pixel about 9
 // times per tile, and the load path, not the matrix cores, sets its time.
-// Here an M tile is BM consecutive output pixels of ONE image, and for each
-// channel chunk (128 bytes of channels) the input rows those pixels need --
-// the halo, (rows spanned - 1) * stride + (KH - 1) * dilation + 1 rows of HC
-// slots, HC = (Wo - 1) * hstride + (KW - 1) * hdilation + 1, a slot per input
-// column from -hpadding on, zero where it is padding -- is loaded into LDS
-// once.  The KH*KW taps of the chunk then read their A operands from that
+// Here an M tile is whole output rows of ONE image (hrows rows, when a row
+// fits the BM tile rows), or one of hcblk pieces of a row, and for each
+// channel chunk (128 bytes of channels) the input pixels those outputs need --
+// the halo, (rows - 1) * stride + (KH - 1) * dilation + 1 input rows of
+// HC = (cols - 1) * hstride + (KW - 1) * hdilation + 1 slots, a slot per input
+// column from the tile's first one on, zero where it is padding -- is loaded
+// into LDS once.  Tiles of whole rows keep the halo free of the partial rows
+// a run of BM pixels would straddle (the tile rows past rows * Wo idle).  The KH*KW taps of the chunk then read their A operands from that
 // buffer at tap-dependent offsets; only the weight slices (BN rows of 128
 // bytes per tap) stream through NS LDS stages as in the pipelined kernel.
 //
@@ -58,16 +60,26 @@ __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &
 
     // ---- the tile's pixels and its halo ----
     const int HoWo = p.Ho * p.Wo;
-    const int tpi = (HoWo + BM - 1) / BM;  // M tiles per image
-    const int b = mt / tpi;
-    const int q0 = (mt - b * tpi) * BM;    // first output pixel of the tile within image b
-    const int q1 = min(q0 + BM, HoWo) - 1;
-    const int r_first = fdiv(q0, x.wo), r_last = fdiv(q1, x.wo);
+    const int CBK = p.hcblk;
+    const int b = mt / p.htpi;
+    const int tin = mt - b * p.htpi;  // tile within the image
+    int trow = tin, tcol = 0;
+    if (CBK > 1) {  // pieces of a row (uniform)
+        trow = tin / CBK;
+        tcol = tin - trow * CBK;
+    }
+    const int r_first = trow * p.hrows, c_first = tcol * BM;  // c_first is 0 for tiles of whole rows
+    const int nrows = min(p.hrows, p.Ho - r_first);
+    const int ncols = CBK == 1 ? p.Wo : min(BM, p.Wo - c_first);
+    const int count = nrows * ncols;        // output pixels of the tile: q0 .. q0 + count - 1 of image b
+    const int q0 = r_first * p.Wo + c_first;
     const int ih0 = r_first * p.vs - p.vp;  // input row of halo row 0
-    const int HC = (p.Wo - 1) * p.hs + (p.KW - 1) * p.hd + 1;
-    const int HR = (r_last - r_first) * p.vs + (p.KH - 1) * p.vd + 1;
+    const int iw0 = c_first * p.hs - p.hp;  // input column of halo column 0
+    // halo rows are HC slots apart, HC that of a full tile (a partial tile loads columns it never reads)
+    const int HC = p.hpitch;
+    const int HR = (nrows - 1) * p.vs + (p.KH - 1) * p.vd + 1;
     const int slots = HR * HC;
-    const FastDiv fhc = make_fastdiv(HC);
+    const FastDiv fhc = host_fastdiv(p.fdm_hp, p.fds_hp, HC);
 
     const __amdgpu_buffer_rsrc_t rA = make_rsrc(p.A, p.a_bytes);
     const __amdgpu_buffer_rsrc_t rB = make_rsrc(p.Bm, p.b_bytes);
@@ -84,7 +96,7 @@ __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &
         const int chunk = lpos ^ ((s >> 1) & 7);
         h_cb[k] = chunk * 16;
         const int hr = fdiv(s, fhc), hc = s - hr * HC;
-        const int ih = ih0 + hr, iw = hc - p.hp;
+        const int ih = ih0 + hr, iw = iw0 + hc;
         const bool ok = s < slots && (unsigned)ih < (unsigned)p.H && (unsigned)iw < (unsigned)p.W;
         h_off[k] = ok ? (unsigned)(((b * p.H + ih) * p.W + iw) * CB + h_cb[k]) : OOB;
     }
@@ -163,11 +175,11 @@ __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &
     int aslot[TR];  // halo slot of this lane's output pixel in each 32-row block (tap 0)
 #pragma unroll
     for (int a = 0; a < TR; a++) {
-        const int q = q0 + wm * WTM + a * 32 + l32;
+        const int m = wm * WTM + a * 32 + l32;
         aslot[a] = 0;  // rows past the tile read slot 0 and are not stored
-        if (q <= q1) {
-            const int orow = fdiv(q, x.wo), ocol = q - orow * p.Wo;
-            aslot[a] = (orow - r_first) * p.vs * HC + ocol * p.hs;
+        if (m < count) {
+            const int orow = CBK == 1 ? fdiv(m, x.wo) : 0, ocol = m - orow * ncols;  // ncols == Wo for whole rows
+            aslot[a] = orow * p.vs * HC + ocol * p.hs;
         }
     }
     f32x16 acc[TR][TC];
@@ -253,13 +265,22 @@ __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &
     // operands of one group, multiplying the previous one and sending a share
     // of the weight loads of step i + NS - 1.  Set 1 carries the last group of
     // a step into the next.
+    // compute-side tap state of the current step: tap index, column of the tap
+    // in the kernel, and its halo offset kx * vd * HC + ky * hd
+    int c_tap = 0, c_ky = 0, c_off = 0, c_ch = 0;  // c_ch: channel chunk of the step, from c_begin
+    const int row_step = p.vd * HC - (p.KW - 1) * p.hd;  // offset change from (kx, KW - 1) to (kx + 1, 0)
+    auto next_tap = [&]() __attribute__((always_inline)) {  // selects, no branches in the step
+        const bool wrap_tap = c_tap + 1 == T_, wrap_ky = c_ky + 1 == p.KW;
+        c_off = wrap_tap ? 0 : c_off + (wrap_ky ? row_step : p.hd);
+        c_ky = wrap_tap || wrap_ky ? 0 : c_ky + 1;
+        c_tap = wrap_tap ? 0 : c_tap + 1;
+        c_ch += wrap_tap ? 1 : 0;
+    };
     auto stage_mma = [&](auto s, auto issue_next, int i) __attribute__((always_inline)) {
         constexpr bool ISS = decltype(issue_next)::value;
         constexpr int SN = (decltype(s)::value + NS - 1) % NS;
         const bool live = i + NS - 1 < nsteps;
-        const int tap = i - (i / T_) * T_;
-        const int kx = tap / p.KW, ky = tap - kx * p.KW;
-        const int tapoff = kx * p.vd * HC + ky * p.hd;
+        const int tapoff = c_off;
         auto region = [&](auto g) __attribute__((always_inline)) {
             constexpr int G = decltype(g)::value;
             __builtin_amdgcn_sched_barrier(0);
@@ -282,6 +303,7 @@ __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &
         }
         __builtin_amdgcn_sched_barrier(0);
         if constexpr (ISS) advance();
+        next_tap();
     };
     // Before step i: the weight loads of step i have landed (those of the NS-2
     // later steps may be in flight) and every wave's LDS reads of step i-1 are
@@ -293,8 +315,8 @@ __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &
         wait_vm<decltype(pending)::value>();
         __builtin_amdgcn_s_waitcnt((0xF) | (0x7 << 4) | (0x0 << 8) | (0x3 << 14));  // lgkmcnt(0)
         __builtin_amdgcn_s_barrier();
-        if (i > 0 && i - (i / T_) * T_ == 0) {  // uniform
-            issue_halo(c_begin + i / T_);
+        if (i > 0 && c_tap == 0) {  // first step of a chunk (uniform)
+            issue_halo(c_begin + c_ch);
             wait_vm<0>();
             __builtin_amdgcn_s_barrier();
         }
@@ -341,7 +363,7 @@ __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &
         mma(r1);
     }
 
-    // ---- write-back: rows = output pixels q0 .. q1 of image b, cols = kn ----
+    // ---- write-back: rows = output pixels q0 .. q0 + count - 1 of image b, cols = kn ----
     const int pbase = b * HoWo + q0;
     if (p.splits > 1) {  // raw split-K partials
 #pragma unroll
@@ -352,7 +374,7 @@ __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &
                 for (int v = 0; v < 16; v++) {
                     const int ml = wm * WTM + a * 32 + 8 * (v >> 2) + 4 * h + (v & 3), nl = wn * WTN + c * 32 + l32;
                     const int n = nt * BN + nl;
-                    if (q0 + ml <= q1 && n < p.KN)
+                    if (ml < count && n < p.KN)
                         p.ws[((long long)split * p.M + pbase + ml) * p.N + n] = acc[a][c][v];
                 }
         return;
@@ -366,7 +388,7 @@ __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &
                 for (int v = 0; v < 16; v++) {
                     const int ml = wm * WTM + a * 32 + 8 * (v >> 2) + 4 * h + (v & 3), nl = wn * WTN + c * 32 + l32;
                     const int n = nt * BN + nl;
-                    if (q0 + ml <= q1 && n < p.KN) store_out<CG_FWD_NHWC>(p, x, pbase + ml, n, acc[a][c][v]);
+                    if (ml < count && n < p.KN) store_out<CG_FWD_NHWC>(p, x, pbase + ml, n, acc[a][c][v]);
                 }
         return;
     }
@@ -389,7 +411,7 @@ __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &
                                          nl = wn * WTN + c * 32 + l32;
                                          m = pbase + ml;
                                          n = nt * BN + nl;
-                                         return q0 + ml <= q1 && n < p.KN;
+                                         return ml < count && n < p.KN;
                                      });
     if (s_epi) __syncthreads();  // the LDS is rewritten by the next item of a persistent launch
 }

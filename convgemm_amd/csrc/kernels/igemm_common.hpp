@@ -95,12 +95,21 @@ struct Ctx {
     int Mrows;       // valid logical rows of this block's problem/class
 };
 
+// The host's multiply-and-shift constants (cg_kparams.fdm_*, fds_*).
+__device__ inline FastDiv host_fastdiv(unsigned m, int s, int d) {
+    FastDiv f;
+    f.m = m;
+    f.s = (uint32_t)s;
+    f.d = d;
+    return f;
+}
+
 __device__ inline Ctx make_ctx(const cg_kparams &p, int cls) {
     Ctx x;
     int op = p.op;
-    x.howo = make_fastdiv(p.Ho * p.Wo > 0 ? p.Ho * p.Wo : 1);
-    x.wo = make_fastdiv(p.Wo > 0 ? p.Wo : 1);
-    x.c = make_fastdiv(p.C > 0 ? p.C : 1);
+    x.howo = host_fastdiv(p.fdm_howo, p.fds_howo, p.Ho * p.Wo);
+    x.wo = host_fastdiv(p.fdm_wo, p.fds_wo, p.Wo);
+    x.c = host_fastdiv(p.fdm_c, p.fds_c, p.C);
     x.rh = 0;
     x.rw = 0;
     x.Hc = p.Ho;
@@ -116,8 +125,13 @@ __device__ inline Ctx make_ctx(const cg_kparams &p, int cls) {
     } else {
         x.Mrows = (int)p.P;
     }
-    x.cls_hw = make_fastdiv(x.Hc * x.Wc > 0 ? x.Hc * x.Wc : 1);
-    x.cls_w = make_fastdiv(x.Wc > 0 ? x.Wc : 1);
+    if (op == CG_DGRAD_NHWC || op == CG_DGRAD_NCHW) {  // class extents differ from Ho, Wo
+        x.cls_hw = make_fastdiv(x.Hc * x.Wc > 0 ? x.Hc * x.Wc : 1);
+        x.cls_w = make_fastdiv(x.Wc > 0 ? x.Wc : 1);
+    } else {
+        x.cls_hw = x.howo;
+        x.cls_w = x.wo;
+    }
     return x;
 }
 

@@ -453,16 +453,31 @@ __global__ void __launch_bounds__(WM * WN * 64) lp_grouped_kernel(const cg_kpara
 template <typename T, int BM, int BN, int WM, int WN, int NS>
 inline int launch_lp(const cg_kparams *kp, const cg_group *g, int *occ, hipStream_t st) {
     constexpr int NT = WM * WN * 64;
+    // FP8 wave tiles of 128x64 and up hold 128 accumulator registers beside
+    // 32-byte operands: their stream-K form spills, and is not built
+    constexpr bool kSK = !(sizeof(T) == 1 && (BM / WM) * (BN / WN) >= 128 * 64);
     if (kp->op != CG_FWD_NHWC || !kp->vec) return (int)hipErrorInvalidValue;
     if (occ) {
-        const void *k = kp->sk < 0 ? reinterpret_cast<const void *>(&lp_sk_kernel<T, BM, BN, WM, WN, NS>)
-                                   : reinterpret_cast<const void *>(&lp_grouped_kernel<T, BM, BN, WM, WN, NS>);
-        return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k, NT, 0);
+        if (kp->sk < 0) {
+            if constexpr (!kSK) {
+                *occ = 0;
+                return 0;
+            } else {
+                return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    occ, reinterpret_cast<const void *>(&lp_sk_kernel<T, BM, BN, WM, WN, NS>), NT, 0);
+            }
+        }
+        return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            occ, reinterpret_cast<const void *>(&lp_grouped_kernel<T, BM, BN, WM, WN, NS>), NT, 0);
     }
     if (kp->sk > 0 && !g) {
-        hipLaunchKernelGGL((lp_sk_kernel<T, BM, BN, WM, WN, NS>), dim3((unsigned)(kp->sk_dp + kp->sk)), dim3(NT), 0,
-                           st, *kp);
-        return (int)hipGetLastError();
+        if constexpr (!kSK) {
+            return (int)hipErrorInvalidValue;
+        } else {
+            hipLaunchKernelGGL((lp_sk_kernel<T, BM, BN, WM, WN, NS>), dim3((unsigned)(kp->sk_dp + kp->sk)), dim3(NT),
+                               0, st, *kp);
+            return (int)hipGetLastError();
+        }
     }
     if (g) {
         if (g->total <= 0) return 0;

@@ -65,10 +65,12 @@ template <> struct Tiles<float> {
         {128, 128, 32, 2, 4, 3, FAM_LP, false},   // 29 8 waves
         {256, 64, 32, 4, 2, 3, FAM_LP, false},    // 30 8 waves
         {64, 64, 64, 2, 2, 0, FAM_GEN, false},    // 31 long K step (weight gradient: fewer barriers per MFMA)
-        {128, 64, 32, 2, 2, 3, FAM_HT, false, 384},   // 32 halo tile
-        {256, 64, 32, 4, 1, 2, FAM_HT, false, 512},   // 33 halo tile
-        {128, 128, 32, 2, 2, 2, FAM_HT, false, 384},  // 34 halo tile
-        {64, 64, 32, 2, 2, 3, FAM_HT, false, 256},    // 35 halo tile
+        {256, 64, 32, 4, 1, 4, FAM_HT, false, 352},   // 32 halo tile
+        {128, 128, 32, 2, 2, 3, FAM_HT, false, 256},  // 33 halo tile
+        {128, 64, 32, 2, 2, 4, FAM_HT, false, 256},   // 34 halo tile
+        {64, 128, 32, 2, 2, 4, FAM_HT, false, 128},   // 35 halo tile
+        {64, 64, 32, 2, 2, 4, FAM_HT, false, 288},    // 36 halo tile
+        {256, 128, 32, 4, 2, 3, FAM_HT, false, 384},  // 37 halo tile
     };
 };
 
@@ -105,10 +107,12 @@ template <> struct Tiles<uint16_t> {
         {256, 64, 64, 4, 2, 3, FAM_LP, false},    // 28 8 waves
         {128, 128, 64, 2, 4, 4, FAM_LP, false},   // 29 8 waves
         {64, 64, 128, 2, 2, 0, FAM_GEN, false},   // 30 long K step (weight gradient: fewer barriers per MFMA)
-        {128, 64, 64, 2, 2, 3, FAM_HT, false, 384},   // 31 halo tile
-        {256, 64, 64, 4, 1, 2, FAM_HT, false, 512},   // 32 halo tile
-        {128, 128, 64, 2, 2, 2, FAM_HT, false, 384},  // 33 halo tile
-        {64, 64, 64, 2, 2, 3, FAM_HT, false, 256},    // 34 halo tile
+        {256, 64, 64, 4, 1, 4, FAM_HT, false, 352},   // 31 halo tile
+        {128, 128, 64, 2, 2, 3, FAM_HT, false, 256},  // 32 halo tile
+        {128, 64, 64, 2, 2, 4, FAM_HT, false, 256},   // 33 halo tile
+        {64, 128, 64, 2, 2, 4, FAM_HT, false, 128},   // 34 halo tile
+        {64, 64, 64, 2, 2, 4, FAM_HT, false, 288},    // 35 halo tile
+        {256, 128, 64, 4, 2, 3, FAM_HT, false, 384},  // 36 halo tile
     };
 };
 
@@ -143,10 +147,12 @@ template <> struct Tiles<uint8_t> {
         {512, 64, 128, 8, 1, 2, FAM_LP, false},   // 26 8 waves
         {256, 64, 128, 4, 2, 3, FAM_LP, false},   // 27 8 waves
         {128, 128, 128, 2, 4, 4, FAM_LP, false},  // 28 8 waves
-        {128, 64, 128, 2, 2, 3, FAM_HT, false, 384},   // 29 halo tile
-        {256, 64, 128, 4, 1, 2, FAM_HT, false, 512},   // 30 halo tile
-        {128, 128, 128, 2, 2, 2, FAM_HT, false, 384},  // 31 halo tile
-        {64, 64, 128, 2, 2, 3, FAM_HT, false, 256},    // 32 halo tile
+        {256, 64, 128, 4, 1, 4, FAM_HT, false, 352},  // 29 halo tile
+        {128, 128, 128, 2, 2, 3, FAM_HT, false, 256}, // 30 halo tile
+        {128, 64, 128, 2, 2, 4, FAM_HT, false, 256},  // 31 halo tile
+        {64, 128, 128, 2, 2, 4, FAM_HT, false, 128},  // 32 halo tile
+        {64, 64, 128, 2, 2, 4, FAM_HT, false, 288},   // 33 halo tile
+        {256, 128, 128, 4, 2, 3, FAM_HT, false, 384}, // 34 halo tile
     };
 };
 

@@ -4,8 +4,8 @@ Its M tiles are runs of output pixels of one image, and the input rows those
 pixels read (the halo) are loaded into LDS once per channel chunk; the taps
 read their operands from it at shifted offsets.  Checks:
 * every halo configuration, forced on shapes with padding, stride, dilation,
-  5x5 and 7x7 kernels, images smaller than a tile, tiles that straddle output
-  rows and channel counts that end inside a 128-byte chunk, with split-K 1-3
+  5x5 and 7x7 kernels, images smaller than a tile, tiles of several output
+  rows and of pieces of a row, channel counts that end inside a 128-byte chunk, with split-K 1-3
   and the fused epilogue, against the FP64 oracle on the dequantized operands;
 * the plan falls back to another kernel exactly when the largest halo of the
   problem does not fit the configuration's slots (counted here by brute force
@@ -40,6 +40,8 @@ SHAPES = [
     Shape(2, 6, 30, 24, 32, 3, 3, 0, 1, 1, 1, 1, 1),    # wide rows, no vertical padding
     Shape(4, 28, 28, 64, 64, 3, 3, 1, 1, 1, 1, 1, 1),   # several tiles per image, split rows
     Shape(2, 20, 20, 256, 32, 3, 3, 1, 1, 1, 1, 1, 1),  # many channel chunks
+    Shape(1, 5, 150, 16, 24, 3, 3, 1, 1, 1, 1, 1, 1),   # rows wider than a tile: pieces of a row
+    Shape(1, 6, 301, 32, 16, 1, 3, 0, 1, 2, 2, 1, 1),   # 1x3, stride 2, pieces of a row
 ]
 
 
@@ -49,17 +51,26 @@ def halo_cfgs(elem):
     return [(i, t[0], t[7]) for i, t in enumerate(cg.tile_configs(elem)[:ns]) if t[7] > 0]
 
 
+def halo_tiles(s: Shape, bm):
+    """Output pixels (indices within an image) of each halo tile: whole output
+    rows when a row fits the bm tile rows, else bm-column pieces of a row."""
+    if s.wo <= bm:
+        r = bm // s.wo
+        return [np.arange(r0 * s.wo, min(r0 + r, s.ho) * s.wo) for r0 in range(0, s.ho, r)]
+    return [np.arange(row * s.wo + c0, row * s.wo + min(c0 + bm, s.wo))
+            for row in range(s.ho) for c0 in range(0, s.wo, bm)]
+
+
 def halo_fits(s: Shape, bm, slots):
-    """Largest halo over the M tiles of an image, counted by listing the input
+    """Largest halo over the tiles of an image, counted by listing the input
     rows and columns (padding included) the tile's pixels read."""
     if s.kh * s.kw == 1:
         return False
-    howo = s.ho * s.wo
     worst = 0
-    for q0 in range(0, howo, bm):
-        qs = np.arange(q0, min(q0 + bm, howo))
+    for qs in halo_tiles(s, bm):
+        assert 0 < len(qs) <= bm
         rows = (qs // s.wo)[:, None] * s.vstr + np.arange(s.kh)[None, :] * s.vdil
-        cols = np.arange(s.wo)[:, None] * s.hstr + np.arange(s.kw)[None, :] * s.hdil
+        cols = (qs % s.wo)[:, None] * s.hstr + np.arange(s.kw)[None, :] * s.hdil
         worst = max(worst, (rows.max() - rows.min() + 1) * (cols.max() - cols.min() + 1))
     return worst <= slots
 
@@ -112,8 +123,7 @@ def test_halo_configurations(dtype, si):
             assert (used == cfg) == fits, (cfg, used, fits)
             served += used == cfg
             assert rel_err(Y.cpu().numpy(), ref)[0] <= TOL[dtype], (cfg, splits)
-    if si != 5:
-        assert served, "no halo configuration served the shape"
+    assert served, "no halo configuration served the shape"
 
 
 @pytest.mark.parametrize("dtype", ["f32", "bf16"])
@@ -159,12 +169,15 @@ def test_halo_groups(dtype, flags):
         for splits in (1, 2):
             rng = np.random.default_rng(500 + cfg * 7 + splits)
             cases = []
-            for i in range(5):
-                h = int(rng.choice([6, 9, 12]))
+            while len(cases) < 5:
+                i = len(cases)
+                b, h = int(rng.choice([1, 2, 3])), int(rng.choice([6, 9, 12]))
                 st = int(rng.choice([1, 2]))
                 c = int(rng.choice(range(16, 161, 16)))
                 kn = int(rng.choice(range(16, 161, 16)))
-                cases.append(Case(D(cg.FWD, NHWC, int(rng.choice([1, 2, 3])), h, h, c, kn, 3, s=st), elem,
+                if not halo_fits(Shape(b, h, h, c, kn, 3, 3, 1, 1, st, st, 1, 1), bm, slots):
+                    continue
+                cases.append(Case(D(cg.FWD, NHWC, b, h, h, c, kn, 3, s=st), elem,
                                   600 + i, alpha=0.75 if i % 4 == 1 else 1.0, beta=1.0 if i % 4 == 3 else 0.0,
                                   epi=BN_RELU if i % 2 == 0 else None))
             print(f"halo group {dtype} cfg {cfg} splits {splits} flags {flags}", flush=True)

@@ -341,7 +341,10 @@ def test_pipelined_stream_k(dtype, layout):
                 cg.conv_ex(d, Wd, Xd, Y, dtype=elem, scales=scales, bias=bias_d, relu=True)
                 torch.cuda.synchronize()
                 plan = cg.last_plan()
-                assert plan[0] == cfg and plan[1] == 0, plan
+                # FP8 wave tiles of 128x64 and up have no stream-K form: plain tiles
+                t = cg.tile_configs(elem)[cfg]
+                no_sk = dtype == "fp8" and (t[0] // t[3]) * (t[1] // t[4]) >= 128 * 64
+                assert plan[0] == cfg and plan[1] == (1 if no_sk else 0), plan
             finally:
                 cg.force_config(-1, 0)
             assert rel_err(Y.cpu().numpy(), ref)[0] <= tol, (si, cfg)
