@@ -87,7 +87,7 @@ class Problem(ctypes.Structure):
                 ("scales", ctypes.c_float * 2)]
 
 
-BATCH_GRAPH, BATCH_PERSISTENT, BATCH_STATIC_WEIGHTS = 1, 2, 4
+BATCH_GRAPH, BATCH_PERSISTENT, BATCH_STATIC_WEIGHTS, BATCH_BOUNDED = 1, 2, 4, 8
 
 _lib = None
 
@@ -565,7 +565,7 @@ def problem(desc: Desc, in_, x, out, alpha=1.0, beta=0.0, bias=None, mean=None, 
 class Batch:
     """convgemm_batch_t: many independent convolutions in grouped launches,
     optionally replayed from a hipGraph (flags BATCH_GRAPH, BATCH_PERSISTENT,
-    BATCH_STATIC_WEIGHTS).  Keep the operand tensors alive while the batch
+    BATCH_STATIC_WEIGHTS, BATCH_BOUNDED).  Keep the operand tensors alive while the batch
     is used: their addresses are bound at creation."""
 
     def __init__(self, problems, dtype=F32, flags=0, keep=None):

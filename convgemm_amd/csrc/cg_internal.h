@@ -122,6 +122,8 @@ int cg_launch_igemm_grouped(int elem, int cfg, const cg_kparams *kp0, const cg_g
 /* Grouped split-K reduction of problems sharing the operation `op`; items
  * are 256-element blocks, cg_reduce_items() per problem. */
 int cg_launch_splitk_reduce_grouped(int op, const cg_group *g, void *stream);
+/* The same over problems of any operations (each item switches on its problem's op). */
+int cg_launch_splitk_reduce_mixed(const cg_group *g, void *stream);
 long long cg_reduce_items(const cg_kparams *kp);
 /* Resident workgroups per CU of a configuration (occupancy), or 0. */
 int cg_igemm_occupancy(int elem, int cfg, const cg_kparams *kp0);
@@ -161,6 +163,24 @@ int cg_launch_scale(float *out, long long n, float beta, void *stream);
  * scale (CG_ELEM_FP8; `scratch`: 8 device bytes, the scale lands at +4). */
 int cg_launch_quantize(const float *src, void *dst, long long n, int elem, void *scratch, void *stream);
 int cg_launch_permute3(const void *src, void *dst, int d0, int d1, int d2, int perm, int esize, void *stream);
+
+/* Element-wise jobs of a batch run in one launch: weight repacks (kind 0, 1:
+ * cg_launch_permute3's perm 0, 1 of [d0][d1][d2] on esize-byte elements) and
+ * beta scaling of n FP32 outputs (kind 2: dst[i] = beta * dst[i], 0 if beta
+ * == 0).  Job j owns items [start[j], start[j + 1]), cg_ew_items() of them
+ * (32x32 transpose tiles, or blocks of CG_EW_BLOCK elements).  jobs and start
+ * are device arrays. */
+typedef struct {
+    const void *src;
+    void *dst;
+    int kind, d0, d1, d2;
+    long long n;
+    float beta;
+} cg_ewjob;
+#define CG_EW_BLOCK 1024
+long long cg_ew_items(const cg_ewjob *job);
+int cg_launch_ew_grouped(const cg_ewjob *jobs, const int *start, int njobs, int total, int grid, int esize,
+                         void *stream);
 
 /* ---- static device scratch of the allocation-free entry point ---- */
 #define CG_SCRATCH_BYTES (32 << 20)

@@ -6,11 +6,18 @@
  * quick-return rules and planner), then sorts the problems by kernel
  * instance (tile configuration, operation, vector loads) and, inside an
  * instance, by K tiles per workgroup item, longest first.  Each instance is
- * one grouped launch over the tiles of all its problems; split-K problems
- * share one grouped reduction per operation.  Every problem's item range
- * starts at a multiple of 8 so the kernels can deal each problem's tiles to
- * the 8 XCDs in contiguous runs.  One device allocation holds the problem
- * tables, the repacked weights and the split-K partials.
+ * one grouped launch over the tiles of all its problems.  A run is one
+ * element-wise launch (weight repacks and beta-only scalings), the grouped
+ * implicit GEMMs and one split-K reduction over the problems of every
+ * operation.  With CONVGEMM_BATCH_BOUNDED, when the plans name more than
+ * CG_BATCH_MAX_GROUPS instances, the problems of the instances with the
+ * least work move to the configuration of the largest instance of their
+ * operation that serves them until the count fits, so a run is at most
+ * CG_BATCH_MAX_GROUPS + 2 kernels.  Every
+ * problem's item range starts at a multiple of 8 so the kernels can deal each
+ * problem's tiles to the 8 XCDs in contiguous runs.  One device allocation
+ * holds the problem and job tables, the repacked weights and the split-K
+ * partials.
  */
 #include <limits.h>
 #include <stdint.h>
@@ -25,6 +32,7 @@
 #include "cg_host.h"
 
 #define AL256(x) (((size_t)(x) + 255) & ~(size_t)255)
+#define CG_BATCH_MAX_GROUPS 16
 
 typedef struct {
     cg_kparams kp;
@@ -43,30 +51,28 @@ typedef struct {
 } glaunch;
 
 typedef struct {
-    int op;
-    cg_group g;
-} rlaunch;
-
-typedef struct {
-    const void *src;
-    void *dst;
-    int d0, d1, d2, perm;
-} repack_t;
-
-typedef struct {
     float *out;
     long long n;
     float beta;
 } scale_t;
 
+/* A grouped element-wise launch over a device job table. */
+typedef struct {
+    const cg_ewjob *jobs;
+    const int *start;
+    int njobs, total, grid;
+} ewlaunch;
+
 struct convgemm_batch_s {
     int elem;
     unsigned flags;
     void *dmem;
-    int ngl, nrl, nrep, nsc;
+    int ngl, nsc;
     glaunch *gl;
-    rlaunch rl[CG_NUM_OPKINDS];
-    repack_t *rep;
+    int has_red;
+    cg_group red;    /* split-K reductions of every operation */
+    ewlaunch ew_run; /* per run: repacks (unless static) and beta scalings */
+    ewlaunch ew_rep; /* static weights: the repacks, run once at creation */
     scale_t *sc;
     hipGraph_t graph;
     hipGraphExec_t exec;
@@ -81,11 +87,6 @@ static int cmp_prob(const void *a, const void *b) {
     if (x->kp.vec != y->kp.vec) return x->kp.vec - y->kp.vec;
     if (x->cost != y->cost) return x->cost > y->cost ? -1 : 1;
     return 0;
-}
-
-static int cmp_red(const void *a, const void *b) {
-    const cg_kparams *x = (const cg_kparams *)a, *y = (const cg_kparams *)b;
-    return x->op - y->op;
 }
 
 /* Plan one problem.  *kind: 1 = implicit GEMM (bp filled), 2 = beta scaling
@@ -160,22 +161,16 @@ static convgemm_status_t plan_one(const convgemm_problem_t *q, convgemm_dtype_t 
     return CONVGEMM_SUCCESS;
 }
 
-static convgemm_status_t issue(convgemm_batch_t bt, hipStream_t s, int with_repack) {
+static convgemm_status_t issue(convgemm_batch_t bt, hipStream_t s) {
     int e;
-    for (int i = 0; i < bt->nsc; i++)
-        if ((e = cg_launch_scale(bt->sc[i].out, bt->sc[i].n, bt->sc[i].beta, s))) return cg_hip_fail(e, "scale");
-    if (with_repack)
-        for (int i = 0; i < bt->nrep; i++) {
-            const repack_t *r = &bt->rep[i];
-            if ((e = cg_launch_permute3(r->src, r->dst, r->d0, r->d1, r->d2, r->perm, cg_esize(bt->elem), s)))
-                return cg_hip_fail(e, "weight repack");
-        }
+    const ewlaunch *w = &bt->ew_run;
+    if (w->njobs && (e = cg_launch_ew_grouped(w->jobs, w->start, w->njobs, w->total, w->grid, cg_esize(bt->elem), s)))
+        return cg_hip_fail(e, "weight repacks and scalings");
     for (int i = 0; i < bt->ngl; i++)
         if ((e = cg_launch_igemm_grouped(bt->elem, bt->gl[i].cfg, &bt->gl[i].kp0, &bt->gl[i].g, s)))
             return cg_hip_fail(e, "grouped implicit GEMM");
-    for (int i = 0; i < bt->nrl; i++)
-        if ((e = cg_launch_splitk_reduce_grouped(bt->rl[i].op, &bt->rl[i].g, s)))
-            return cg_hip_fail(e, "grouped split-K reduction");
+    if (bt->has_red && (e = cg_launch_splitk_reduce_mixed(&bt->red, s)))
+        return cg_hip_fail(e, "grouped split-K reduction");
     return CONVGEMM_SUCCESS;
 }
 
@@ -185,9 +180,93 @@ static void release(convgemm_batch_t bt) {
     if (bt->graph) hipGraphDestroy(bt->graph);
     if (bt->dmem) hipFree(bt->dmem);
     free(bt->gl);
-    free(bt->rep);
     free(bt->sc);
     free(bt);
+}
+
+static int same_instance(const bprob *x, const bprob *y) {
+    return x->cfg == y->cfg && x->kp.op == y->kp.op && x->kp.vec == y->kp.vec;
+}
+
+/* Move the problems of the least-work instances to the configuration of the
+ * largest instance of their operation and load width that serves all of
+ * them, until at most CG_BATCH_MAX_GROUPS instances remain (or no move is
+ * possible).  bp is sorted by instance; it is sorted again after each move. */
+static void consolidate(int elem, bprob *bp, int np) {
+    for (;;) {
+        int ng = 0;
+        for (int i = 0; i < np; i++)
+            if (i == 0 || !same_instance(&bp[i - 1], &bp[i])) ng++;
+        if (ng <= CG_BATCH_MAX_GROUPS) return;
+        /* instances as [first, end) ranges with their work (items x K tiles) */
+        int *first = (int *)malloc(sizeof(int) * (size_t)(ng + 1));
+        double *work = (double *)calloc((size_t)ng, sizeof(double));
+        char *stuck = (char *)calloc((size_t)ng, 1); /* instances that no sibling serves */
+        if (!first || !work || !stuck) {
+            free(first);
+            free(work);
+            free(stuck);
+            return;
+        }
+        int g = -1;
+        for (int i = 0; i < np; i++) {
+            if (i == 0 || !same_instance(&bp[i - 1], &bp[i])) first[++g] = i;
+            work[g] += (double)bp[i].items * (double)bp[i].cost;
+        }
+        first[ng] = np;
+        int moved = 0;
+        /* the lightest instance that has a heavier sibling serving all its problems */
+        for (int tries = 0; tries < ng && !moved; tries++) {
+            int src = -1;
+            for (int k = 0; k < ng; k++)
+                if (!stuck[k] && (src < 0 || work[k] < work[src])) src = k;
+            if (src < 0) break;
+            int dst = -1;
+            for (int k = 0; k < ng; k++) {
+                if (k == src || bp[first[k]].kp.op != bp[first[src]].kp.op ||
+                    bp[first[k]].kp.vec != bp[first[src]].kp.vec || work[k] < work[src])
+                    continue;
+                int ok = 1;
+                for (int i = first[src]; i < first[src + 1] && ok; i++)
+                    ok = cg_plan_usable(elem, bp[first[k]].cfg, &bp[i].kp);
+                if (ok && (dst < 0 || work[k] > work[dst])) dst = k;
+            }
+            if (dst < 0) {
+                stuck[src] = 1; /* cannot move: try the next lightest */
+                continue;
+            }
+            const int cfg = bp[first[dst]].cfg;
+            cg_tilecfg t;
+            if (cg_get_tilecfg(elem, cfg, &t)) break;
+            for (int i = first[src]; i < first[src + 1]; i++) {
+                bprob *q = &bp[i];
+                cg_fill_tiles(&q->kp, &t, q->kp.splits);
+                q->cfg = cfg;
+                q->pbytes = cg_partial_bytes(&q->kp);
+                q->items = (long long)q->kp.mtiles * q->kp.ntiles * q->kp.splits * q->kp.ncls_h * q->kp.ncls_w;
+                q->cost = (q->kp.KT + q->kp.splits - 1) / q->kp.splits;
+            }
+            moved = 1;
+        }
+        free(first);
+        free(work);
+        free(stuck);
+        if (!moved) return;
+        qsort(bp, (size_t)np, sizeof(bprob), cmp_prob);
+    }
+}
+
+/* Item ranges (cg_ew_items) of njobs jobs into start[];
+ * returns the total, or -1 past 2^31. */
+static long long ew_ranges(const cg_ewjob *jobs, int njobs, int *start) {
+    long long acc = 0;
+    for (int j = 0; j < njobs; j++) {
+        start[j] = (int)acc;
+        acc += cg_ew_items(&jobs[j]);
+        if (acc > INT_MAX) return -1;
+    }
+    start[njobs] = (int)acc;
+    return acc;
 }
 
 convgemm_status_t convgemm_batch_create(const convgemm_problem_t *problems, int n, convgemm_dtype_t dtype,
@@ -195,7 +274,8 @@ convgemm_status_t convgemm_batch_create(const convgemm_problem_t *problems, int 
     if (!batch) return errf(CONVGEMM_ERR_INVALID_VALUE, "batch output pointer is NULL");
     *batch = NULL;
     if (n < 0 || (n > 0 && !problems)) return errf(CONVGEMM_ERR_INVALID_VALUE, "invalid problem list");
-    if (flags & ~(unsigned)(CONVGEMM_BATCH_GRAPH | CONVGEMM_BATCH_PERSISTENT | CONVGEMM_BATCH_STATIC_WEIGHTS))
+    if (flags & ~(unsigned)(CONVGEMM_BATCH_GRAPH | CONVGEMM_BATCH_PERSISTENT | CONVGEMM_BATCH_STATIC_WEIGHTS |
+                            CONVGEMM_BATCH_BOUNDED))
         return errf(CONVGEMM_ERR_INVALID_VALUE, "unknown batch flags");
     if (dtype != CONVGEMM_F32 && dtype != CONVGEMM_BF16 && dtype != CONVGEMM_FP8_E4M3)
         return errf(CONVGEMM_ERR_INVALID_VALUE, "unknown dtype");
@@ -204,6 +284,7 @@ convgemm_status_t convgemm_batch_create(const convgemm_problem_t *problems, int 
     bprob *bp = (bprob *)calloc((size_t)(n > 0 ? n : 1), sizeof(bprob));
     int *hstart = NULL;
     cg_kparams *htab = NULL;
+    cg_ewjob *hjobs = NULL;
     convgemm_status_t st = CONVGEMM_SUCCESS;
     if (!bt || !bp) {
         st = errf(CONVGEMM_ERR_ALLOC, "host allocation failed");
@@ -227,6 +308,7 @@ convgemm_status_t convgemm_batch_create(const convgemm_problem_t *problems, int 
         else if (kind == 2) bt->nsc++;
     }
     qsort(bp, (size_t)np, sizeof(bprob), cmp_prob);
+    if (flags & CONVGEMM_BATCH_BOUNDED) consolidate(bt->elem, bp, np);
 
     /* ---- groups, item ranges, device layout ---- */
     int ngl = 0, nsplit = 0, nrep = 0;
@@ -235,23 +317,20 @@ convgemm_status_t convgemm_batch_create(const convgemm_problem_t *problems, int 
         if (bp[i].repack) nrep++;
     }
     for (int i = 0; i < np; i++)
-        if (i == 0 || bp[i - 1].cfg != bp[i].cfg || bp[i - 1].kp.op != bp[i].kp.op ||
-            bp[i - 1].kp.vec != bp[i].kp.vec)
-            ngl++;
-    int nops = 0;
-    {
-        int seen[CG_NUM_OPKINDS] = {0};
-        for (int i = 0; i < np; i++)
-            if (bp[i].kp.splits > 1 && !seen[bp[i].kp.op]) { seen[bp[i].kp.op] = 1; nops++; }
-    }
+        if (i == 0 || !same_instance(&bp[i - 1], &bp[i])) ngl++;
+    const int static_w = (flags & CONVGEMM_BATCH_STATIC_WEIGHTS) != 0;
+    /* job table: [repacks][scalings]; per run: the repacks unless static, then the scalings */
+    const int njobs = nrep + bt->nsc;
     const size_t tab_bytes = AL256(sizeof(cg_kparams) * (size_t)(np + nsplit));
-    const size_t start_bytes = AL256(sizeof(int) * (size_t)(np + ngl + nsplit + nops));
+    const size_t job_bytes = AL256(sizeof(cg_ewjob) * (size_t)(njobs > 0 ? njobs : 1));
+    const int nstart = np + ngl + nsplit + 1 + (njobs + 1) + (nrep + 1);
+    const size_t start_bytes = AL256(sizeof(int) * (size_t)nstart);
     size_t wtot = 0, ptot = 0;
     for (int i = 0; i < np; i++) {
         wtot += AL256(bp[i].wbytes);
         ptot += AL256(bp[i].pbytes);
     }
-    const size_t total_bytes = tab_bytes + start_bytes + wtot + ptot;
+    const size_t total_bytes = tab_bytes + job_bytes + start_bytes + wtot + ptot;
     if (np > 0 || bt->nsc > 0) {
         int e = cg_malloc(&bt->dmem, total_bytes ? total_bytes : 256);
         if (e) {
@@ -262,25 +341,29 @@ convgemm_status_t convgemm_batch_create(const convgemm_problem_t *problems, int 
     }
     char *base = (char *)bt->dmem;
     cg_kparams *dtab = (cg_kparams *)base;
-    int *dstart = (int *)(base + tab_bytes);
-    char *wptr = base + tab_bytes + start_bytes;
+    cg_ewjob *djobs = (cg_ewjob *)(base + tab_bytes);
+    int *dstart = (int *)(base + tab_bytes + job_bytes);
+    char *wptr = base + tab_bytes + job_bytes + start_bytes;
     char *pptr = wptr + wtot;
 
     bt->gl = (glaunch *)calloc((size_t)(ngl > 0 ? ngl : 1), sizeof(glaunch));
-    bt->rep = (repack_t *)calloc((size_t)(nrep > 0 ? nrep : 1), sizeof(repack_t));
     htab = (cg_kparams *)calloc((size_t)(np + nsplit > 0 ? np + nsplit : 1), sizeof(cg_kparams));
-    hstart = (int *)calloc((size_t)(np + ngl + nsplit + nops > 0 ? np + ngl + nsplit + nops : 1), sizeof(int));
-    if (!bt->gl || !bt->rep || !htab || !hstart) {
+    hstart = (int *)calloc((size_t)nstart, sizeof(int));
+    hjobs = (cg_ewjob *)calloc((size_t)(njobs > 0 ? njobs : 1), sizeof(cg_ewjob));
+    if (!bt->gl || !htab || !hstart || !hjobs) {
         st = errf(CONVGEMM_ERR_ALLOC, "host allocation failed");
         goto fail;
     }
-    /* workspaces */
+    /* workspaces and element-wise jobs */
+    int nj = 0;
     for (int i = 0; i < np; i++) {
         if (bp[i].repack) {
-            repack_t *r = &bt->rep[bt->nrep++];
-            r->src = bp[i].wsrc;
-            r->dst = wptr;
-            r->d0 = bp[i].d0; r->d1 = bp[i].d1; r->d2 = bp[i].d2; r->perm = bp[i].perm;
+            cg_ewjob *j = &hjobs[nj++];
+            j->src = bp[i].wsrc;
+            j->dst = wptr;
+            j->kind = bp[i].perm ? 1 : 0;
+            j->d0 = bp[i].d0; j->d1 = bp[i].d1; j->d2 = bp[i].d2;
+            j->n = (long long)bp[i].d0 * bp[i].d1 * bp[i].d2;
             bp[i].kp.Bm = wptr;
             wptr += AL256(bp[i].wbytes);
         }
@@ -290,6 +373,17 @@ convgemm_status_t convgemm_batch_create(const convgemm_problem_t *problems, int 
         }
         htab[i] = bp[i].kp;
     }
+    for (int i = 0; i < bt->nsc; i++) {
+        cg_ewjob *j = &hjobs[nj++];
+        j->dst = bt->sc[i].out;
+        j->kind = 2;
+        j->n = bt->sc[i].n;
+        j->beta = bt->sc[i].beta;
+        if (j->n > INT_MAX) {
+            st = errf(CONVGEMM_ERR_UNSUPPORTED, "a batch output has more than 2^31 elements");
+            goto fail;
+        }
+    }
     /* implicit-GEMM groups: item ranges padded to multiples of 8 */
     int dev = 0, ncu = 0;
     hipGetDevice(&dev);
@@ -297,7 +391,7 @@ convgemm_status_t convgemm_batch_create(const convgemm_problem_t *problems, int 
     int so = 0; /* offset into hstart */
     for (int i = 0; i < np;) {
         int j = i;
-        while (j < np && bp[j].cfg == bp[i].cfg && bp[j].kp.op == bp[i].kp.op && bp[j].kp.vec == bp[i].kp.vec) j++;
+        while (j < np && same_instance(&bp[j], &bp[i])) j++;
         glaunch *g = &bt->gl[bt->ngl++];
         g->cfg = bp[i].cfg;
         g->kp0 = bp[i].kp;
@@ -326,57 +420,79 @@ convgemm_status_t convgemm_batch_create(const convgemm_problem_t *problems, int 
         so += j - i + 1;
         i = j;
     }
-    /* split-K reductions: one grouped launch per operation */
+    /* split-K reductions of every operation: one grouped launch */
     int nr = 0;
     for (int i = 0; i < np; i++)
         if (bp[i].kp.splits > 1) htab[np + nr++] = bp[i].kp;
-    qsort(htab + np, (size_t)nr, sizeof(cg_kparams), cmp_red);
-    for (int i = 0; i < nr;) {
-        int j = i;
-        while (j < nr && htab[np + j].op == htab[np + i].op) j++;
-        rlaunch *r = &bt->rl[bt->nrl++];
-        r->op = htab[np + i].op;
+    if (nr > 0) {
         long long acc = 0;
-        for (int k = i; k < j; k++) {
-            hstart[so + (k - i)] = (int)acc;
+        for (int k = 0; k < nr; k++) {
+            hstart[so + k] = (int)acc;
             acc += cg_reduce_items(&htab[np + k]);
             if (acc > INT_MAX) {
                 st = errf(CONVGEMM_ERR_UNSUPPORTED, "a batch reduction has more than 2^31 blocks");
                 goto fail;
             }
         }
-        hstart[so + (j - i)] = (int)acc;
-        r->g.plist = dtab + np + i;
-        r->g.start = dstart + so;
-        r->g.nprob = j - i;
-        r->g.total = (int)acc;
-        r->g.grid = (int)acc;
-        if ((flags & CONVGEMM_BATCH_PERSISTENT) && r->g.grid > 8 * ncu) r->g.grid = 8 * ncu;
-        so += j - i + 1;
-        i = j;
+        hstart[so + nr] = (int)acc;
+        bt->has_red = 1;
+        bt->red.plist = dtab + np;
+        bt->red.start = dstart + so;
+        bt->red.nprob = nr;
+        bt->red.total = (int)acc;
+        bt->red.grid = (int)acc;
+        if ((flags & CONVGEMM_BATCH_PERSISTENT) && bt->red.grid > 8 * ncu) bt->red.grid = 8 * ncu;
+        so += nr + 1;
     }
-    if (np + nr > 0) {
-        hipError_t e1 = hipMemcpy(dtab, htab, sizeof(cg_kparams) * (size_t)(np + nr), hipMemcpyHostToDevice);
-        hipError_t e2 = hipMemcpy(dstart, hstart, sizeof(int) * (size_t)so, hipMemcpyHostToDevice);
-        if (e1 != hipSuccess || e2 != hipSuccess) {
-            st = cg_hip_fail(e1 != hipSuccess ? e1 : e2, "copy of the batch tables");
+    /* element-wise launches: per run (repacks unless static, scalings), and
+     * the static repacks alone (jobs 0 .. nrep-1, run once below) */
+    {
+        const int first = static_w ? nrep : 0;
+        bt->ew_run.njobs = njobs - first;
+        if (bt->ew_run.njobs > 0) {
+            const long long tot = ew_ranges(hjobs + first, bt->ew_run.njobs, hstart + so);
+            if (tot < 0) {
+                st = errf(CONVGEMM_ERR_UNSUPPORTED, "a batch has more than 2^31 element-wise blocks");
+                goto fail;
+            }
+            bt->ew_run.jobs = djobs + first;
+            bt->ew_run.start = dstart + so;
+            bt->ew_run.total = (int)tot;
+            bt->ew_run.grid = (int)(tot < 8LL * ncu ? tot : 8LL * ncu);
+            so += bt->ew_run.njobs + 1;
+        }
+        if (static_w && nrep > 0) {
+            const long long tot = ew_ranges(hjobs, nrep, hstart + so);
+            if (tot < 0) {
+                st = errf(CONVGEMM_ERR_UNSUPPORTED, "a batch has more than 2^31 element-wise blocks");
+                goto fail;
+            }
+            bt->ew_rep.jobs = djobs;
+            bt->ew_rep.start = dstart + so;
+            bt->ew_rep.njobs = nrep;
+            bt->ew_rep.total = (int)tot;
+            bt->ew_rep.grid = (int)(tot < 8LL * ncu ? tot : 8LL * ncu);
+            so += nrep + 1;
+        }
+    }
+    if (np + nr > 0 || njobs > 0) {
+        hipError_t e1 = hipSuccess, e2 = hipSuccess, e3 = hipSuccess;
+        if (np + nr > 0) e1 = hipMemcpy(dtab, htab, sizeof(cg_kparams) * (size_t)(np + nr), hipMemcpyHostToDevice);
+        if (njobs > 0) e2 = hipMemcpy(djobs, hjobs, sizeof(cg_ewjob) * (size_t)njobs, hipMemcpyHostToDevice);
+        e3 = hipMemcpy(dstart, hstart, sizeof(int) * (size_t)(so > 0 ? so : 1), hipMemcpyHostToDevice);
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+            st = cg_hip_fail(e1 != hipSuccess ? e1 : e2 != hipSuccess ? e2 : e3, "copy of the batch tables");
             goto fail;
         }
     }
 
     /* ---- static weights: repack now, once ---- */
-    if ((flags & CONVGEMM_BATCH_STATIC_WEIGHTS) && bt->nrep) {
-        for (int i = 0; i < bt->nrep; i++) {
-            const repack_t *r = &bt->rep[i];
-            int e = cg_launch_permute3(r->src, r->dst, r->d0, r->d1, r->d2, r->perm, cg_esize(bt->elem), NULL);
-            if (e) {
-                st = cg_hip_fail(e, "weight repack");
-                goto fail;
-            }
-        }
-        hipError_t e = hipStreamSynchronize(NULL);
-        if (e != hipSuccess) {
-            st = cg_hip_fail(e, "weight repack");
+    if (bt->ew_rep.njobs) {
+        const ewlaunch *w = &bt->ew_rep;
+        int e = cg_launch_ew_grouped(w->jobs, w->start, w->njobs, w->total, w->grid, cg_esize(bt->elem), NULL);
+        hipError_t e2 = e ? hipSuccess : hipStreamSynchronize(NULL);
+        if (e || e2 != hipSuccess) {
+            st = cg_hip_fail(e ? e : (int)e2, "weight repack");
             goto fail;
         }
     }
@@ -391,7 +507,7 @@ convgemm_status_t convgemm_batch_create(const convgemm_problem_t *problems, int 
         }
         e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-            st = issue(bt, cs, !(flags & CONVGEMM_BATCH_STATIC_WEIGHTS));
+            st = issue(bt, cs);
             hipError_t e2 = hipStreamEndCapture(cs, &bt->graph);
             if (st == CONVGEMM_SUCCESS && e2 != hipSuccess) st = cg_hip_fail(e2, "graph capture");
             if (st == CONVGEMM_SUCCESS) {
@@ -410,6 +526,7 @@ convgemm_status_t convgemm_batch_create(const convgemm_problem_t *problems, int 
     free(bp);
     free(htab);
     free(hstart);
+    free(hjobs);
     *batch = bt;
     return cg_set_error(CONVGEMM_SUCCESS, "");
 
@@ -417,6 +534,7 @@ fail:
     free(bp);
     free(htab);
     free(hstart);
+    free(hjobs);
     if (bt && bt->dmem) hipDeviceSynchronize();
     release(bt);
     return st;
@@ -429,7 +547,7 @@ convgemm_status_t convgemm_batch_run(convgemm_batch_t bt, convgemm_stream_t stre
         if (e != hipSuccess) return cg_hip_fail(e, "graph launch");
         return cg_set_error(CONVGEMM_SUCCESS, "");
     }
-    convgemm_status_t st = issue(bt, (hipStream_t)stream, !(bt->flags & CONVGEMM_BATCH_STATIC_WEIGHTS));
+    convgemm_status_t st = issue(bt, (hipStream_t)stream);
     if (st != CONVGEMM_SUCCESS) return st;
     return cg_set_error(CONVGEMM_SUCCESS, "");
 }
@@ -437,8 +555,7 @@ convgemm_status_t convgemm_batch_run(convgemm_batch_t bt, convgemm_stream_t stre
 convgemm_status_t convgemm_batch_info(convgemm_batch_t bt, int *groups, int *kernels_per_run, int *graph) {
     if (!bt) return errf(CONVGEMM_ERR_INVALID_VALUE, "batch is NULL");
     if (groups) *groups = bt->ngl;
-    if (kernels_per_run)
-        *kernels_per_run = bt->nsc + ((bt->flags & CONVGEMM_BATCH_STATIC_WEIGHTS) ? 0 : bt->nrep) + bt->ngl + bt->nrl;
+    if (kernels_per_run) *kernels_per_run = (bt->ew_run.njobs > 0) + bt->ngl + bt->has_red;
     if (graph) *graph = bt->exec != NULL;
     return cg_set_error(CONVGEMM_SUCCESS, "");
 }

@@ -78,6 +78,34 @@ __global__ void __launch_bounds__(256) splitk_reduce_grouped_kernel(const cg_kpa
     }
 }
 
+// Grouped over problems of any operations: the op of each item's problem
+// picks the instantiation (uniform per item).
+__global__ void __launch_bounds__(256) splitk_reduce_mixed_kernel(const cg_kparams *__restrict__ plist,
+                                                                  const int *__restrict__ start, int nprob,
+                                                                  int total) {
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        int lo = 0, hi = nprob - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (start[mid] <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const cg_kparams &p = plist[lo];
+        const int local = item - start[lo];
+        const int per = 256 / reduce_groups(p);
+        const int per_cls = (int)(((long long)p.M * p.N + per - 1) / per);
+        const int cls = local / per_cls, blk = local - cls * per_cls;
+        switch (p.op) {
+        case CG_FWD_NHWC: reduce_block<CG_FWD_NHWC>(p, cls, blk); break;
+        case CG_FWD_NCHW: reduce_block<CG_FWD_NCHW>(p, cls, blk); break;
+        case CG_WGRAD_NHWC: reduce_block<CG_WGRAD_NHWC>(p, cls, blk); break;
+        case CG_WGRAD_NCHW: reduce_block<CG_WGRAD_NCHW>(p, cls, blk); break;
+        case CG_DGRAD_NHWC: reduce_block<CG_DGRAD_NHWC>(p, cls, blk); break;
+        default: reduce_block<CG_DGRAD_NCHW>(p, cls, blk); break;
+        }
+    }
+}
+
 // ---- stream-K fix-up: one workgroup per tile of the stream-K region (tiles
 // sk_dp ..); tiles that several stream-K workgroups share sum their pieces
 // (workgroup order, deterministic) and go through the epilogue; whole tiles
@@ -229,6 +257,74 @@ __global__ void permute_short_kernel(const E *__restrict__ src, E *__restrict__ 
     }
 }
 
+// ---- a batch's element-wise jobs (cg_ewjob) in one launch.  A repack is a
+// batch of Z 2-D transposes, src (z, r, c) at z*sz + r*sr + c to dst (z, c, r)
+// at z*tz + c*tc + r, run as 32x32 tiles through LDS (cg_launch_permute3's
+// decomposition); an item is one tile, or CG_EW_BLOCK elements of a scaling.
+struct EwGeom {
+    int R, C, Z;
+    long long sz, sr, tz, tc;
+};
+
+__host__ __device__ inline EwGeom ew_geom(const cg_ewjob &j) {
+    EwGeom g;
+    const long long d0 = j.d0, d1 = j.d1, d2 = j.d2;
+    if (j.kind == 0) {  // [d0][d1][d2] -> [d2][d1][d0]
+        g.R = j.d0; g.C = j.d2; g.Z = j.d1;
+        g.sz = d2; g.sr = d1 * d2; g.tz = d0; g.tc = d1 * d0;
+    } else {            // [d0][d1][d2] -> [d0][d2][d1]
+        g.R = j.d1; g.C = j.d2; g.Z = j.d0;
+        g.sz = d1 * d2; g.sr = d2; g.tz = d2 * d1; g.tc = d1;
+    }
+    return g;
+}
+
+extern "C" long long cg_ew_items(const cg_ewjob *j) {
+    if (j->kind == 2) return (j->n + CG_EW_BLOCK - 1) / CG_EW_BLOCK;
+    const EwGeom g = ew_geom(*j);
+    return (long long)g.Z * ((g.R + 31) / 32) * ((g.C + 31) / 32);
+}
+
+template <typename E>
+__global__ void __launch_bounds__(256) ew_grouped_kernel(const cg_ewjob *__restrict__ jobs,
+                                                         const int *__restrict__ start, int njobs, int total) {
+    __shared__ E t[32][33];
+    const int tx = (int)threadIdx.x & 31, ty = (int)threadIdx.x >> 5;
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        int lo = 0, hi = njobs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (start[mid] <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const cg_ewjob &j = jobs[lo];
+        const int local = item - start[lo];
+        if (j.kind == 2) {
+            float *out = (float *)j.dst;
+            const long long i0 = (long long)local * CG_EW_BLOCK;
+            const long long i1 = min(i0 + CG_EW_BLOCK, j.n);
+            for (long long i = i0 + threadIdx.x; i < i1; i += 256) out[i] = j.beta == 0.0f ? 0.0f : j.beta * out[i];
+            continue;
+        }
+        const EwGeom g = ew_geom(j);
+        const int tr = (g.R + 31) / 32, tcn = (g.C + 31) / 32;
+        const int z = local / (tr * tcn), rem = local - z * (tr * tcn);
+        const int r0 = (rem / tcn) * 32, c0 = (rem - (rem / tcn) * tcn) * 32;
+        const E *src = (const E *)j.src + (long long)z * g.sz;
+        E *dst = (E *)j.dst + (long long)z * g.tz;
+        __syncthreads();  // the previous item's reads of t are done
+        for (int i = ty; i < 32; i += 8) {
+            const int r = r0 + i, c = c0 + tx;
+            if (r < g.R && c < g.C) t[i][tx] = src[(long long)r * g.sr + c];
+        }
+        __syncthreads();
+        for (int i = ty; i < 32; i += 8) {
+            const int c = c0 + i, r = r0 + tx;
+            if (r < g.R && c < g.C) dst[(long long)c * g.tc + r] = t[tx][i];
+        }
+    }
+}
+
 // ---- FP32 -> BF16 (round to nearest even) ----
 __device__ inline uint16_t f32_to_bf16_rne(float f) {
     uint32_t u = __builtin_bit_cast(uint32_t, f);
@@ -363,6 +459,29 @@ extern "C" int cg_launch_splitk_reduce(const cg_kparams *kp, void *stream) {
     case CG_WGRAD_NCHW: launch_reduce<CG_WGRAD_NCHW>(kp, grid, st); break;
     case CG_DGRAD_NHWC: launch_reduce<CG_DGRAD_NHWC>(kp, grid, st); break;
     case CG_DGRAD_NCHW: launch_reduce<CG_DGRAD_NCHW>(kp, grid, st); break;
+    default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
+
+extern "C" int cg_launch_splitk_reduce_mixed(const cg_group *g, void *stream) {
+    if (g->total <= 0) return 0;
+    hipLaunchKernelGGL(splitk_reduce_mixed_kernel, dim3((unsigned)g->grid), dim3(256), 0, (hipStream_t)stream,
+                       g->plist, g->start, g->nprob, g->total);
+    return (int)hipGetLastError();
+}
+
+extern "C" int cg_launch_ew_grouped(const cg_ewjob *jobs, const int *start, int njobs, int total, int grid,
+                                    int esize, void *stream) {
+    if (total <= 0 || njobs <= 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    switch (esize) {
+    case 4: hipLaunchKernelGGL(ew_grouped_kernel<float>, dim3((unsigned)grid), dim3(256), 0, st, jobs, start, njobs,
+                               total); break;
+    case 2: hipLaunchKernelGGL(ew_grouped_kernel<uint16_t>, dim3((unsigned)grid), dim3(256), 0, st, jobs, start,
+                               njobs, total); break;
+    case 1: hipLaunchKernelGGL(ew_grouped_kernel<uint8_t>, dim3((unsigned)grid), dim3(256), 0, st, jobs, start,
+                               njobs, total); break;
     default: return (int)hipErrorInvalidValue;
     }
     return (int)hipGetLastError();

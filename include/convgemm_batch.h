@@ -8,7 +8,9 @@
  * launch whose workgroups walk the tiles of all the group's problems, so that
  * small and uneven layers fill the GPU together.  With
  * CONVGEMM_BATCH_GRAPH the launches are captured once into a hipGraph at
- * creation and every run replays it.
+ * creation and every run replays it.  A run is one element-wise launch
+ * (weight repacks, beta-only scalings), the grouped implicit GEMMs and one
+ * split-K reduction.
  */
 #ifndef CONVGEMM_AMD_CONVGEMM_BATCH_H
 #define CONVGEMM_AMD_CONVGEMM_BATCH_H
@@ -43,7 +45,14 @@ enum {
     /* repack the weights once, at creation: later changes to the weights of
      * problems that need a repack (forward except NCHW 1x1, NCHW backward
      * data) are not seen by runs */
-    CONVGEMM_BATCH_STATIC_WEIGHTS = 4
+    CONVGEMM_BATCH_STATIC_WEIGHTS = 4,
+    /* at most 16 grouped implicit-GEMM launches (18 kernels per run): when the
+     * plans name more kernel configurations, the problems of the groups with
+     * the least work run on the configuration of a larger group of their
+     * operation.  Those problems' outputs then differ from a per-call run in
+     * summation order (within the operand type's accumulation tolerance);
+     * without this flag every output is bit-identical to a per-call run. */
+    CONVGEMM_BATCH_BOUNDED = 8
 };
 
 /* Plan the n problems, allocate the batch's device workspace (repacked

@@ -262,3 +262,61 @@ def test_batch_errors():
     torch.cuda.synchronize()
     assert empty.info()[0] == 0
     empty.close()
+
+
+@pytest.mark.parametrize("dtype", [cg.F32, cg.BF16])
+def test_bounded_batch(dtype, tmp_path):
+    """CONVGEMM_BATCH_BOUNDED on a batch whose plans (set through a tuning
+    database) name 24 kernel instances: at most 16 grouped launches, one
+    element-wise launch and one reduction per run.  Problems that kept their
+    plan are bit-identical to their per-call run; every output matches its
+    per-call run within the accumulation tolerance."""
+    ops = [(cg.FWD, cg.NHWC), (cg.FWD, cg.NCHW), (cg.WGRAD, cg.NHWC), (cg.WGRAD, cg.NCHW), (cg.DGRAD, cg.NHWC),
+           (cg.DGRAD, cg.NCHW)]
+    confs = cg.tile_configs(dtype)
+    ns = cg.num_tile_shapes(dtype)
+    rng = np.random.default_rng(77)
+    cases, lines = [], []
+    for i in range(24):
+        op, layout = ops[i % 6]
+        b, c, kn, st = int(rng.choice([1, 2])), int(rng.choice([16, 32, 48])), int(rng.choice([16, 24, 40])), \
+            int(rng.choice([1, 2]))
+        desc = D(op, layout, b, 10, 10, c, kn, 3, s=st)
+        opk = 2 * [cg.FWD, cg.WGRAD, cg.DGRAD].index(op) + (layout == cg.NCHW)
+        usable = [k for k in range(ns) if confs[k][7] == 0 and cg.tile_supports(dtype, k, op, layout)]
+        cfg, splits = usable[(i // 6) % len(usable)], 1 + (i % 3 == 0)
+        for vec in (0, 1):  # a plan for either load width the problem gets
+            lines.append(f"{int(dtype)} {opk} {vec} {b} 10 10 {c} {kn} 3 3 1 1 {st} {st} 1 1 1 {cfg} {splits} 1.0")
+        cases.append(Case(desc, dtype, 900 + i, beta=1.0 if i % 5 == 4 else 0.0,
+                          epi=BN_RELU if op == cg.FWD else None))
+    db = tmp_path / "plans.tunedb"
+    db.write_text("\n".join(lines) + "\n")
+    prev = cg.get_planner()
+    cg.tune_clear()
+    cg.tune_load(str(db))
+    cg.set_planner(cg.PLAN_TUNED)
+    try:
+        refs, plans = [], set()
+        for c in cases:
+            refs.append(c.reference())
+            plans.add((cg.last_plan()[0], c.desc.op, c.desc.layout))
+        assert len(plans) > 16, plans
+        b = cg.Batch([c.problem() for c in cases], dtype, cg.BATCH_GRAPH | cg.BATCH_BOUNDED)
+    finally:
+        cg.set_planner(prev)
+        cg.tune_clear()
+    groups, kernels, graph = b.info()
+    assert groups <= 16 and kernels <= 18 and graph, (groups, kernels)
+    for rep in range(2):
+        for c in cases:
+            c.reset()
+        b.run()
+        torch.cuda.synchronize()
+        same = 0
+        for i, (c, r) in enumerate(zip(cases, refs)):
+            got, exp = c.out.cpu().numpy(), r.cpu().numpy()
+            scale = max(float(np.max(np.abs(exp))), 1e-30)
+            assert float(np.max(np.abs(got - exp))) <= 1e-5 * scale, (rep, i)
+            same += bool(np.array_equal(got, exp))
+        assert same >= len(cases) // 2, same
+    b.close()
