@@ -227,6 +227,8 @@ def lib() -> ctypes.CDLL:
     L.convgemm_transpose_nchw_ex.restype = ci
     L.cg_set_chunk_window.argtypes = [ctypes.c_longlong]
     L.cg_set_chunk_window.restype = None
+    L.cg_set_dgrad_forward.argtypes = [ci]
+    L.cg_set_dgrad_forward.restype = None
     L.convgemm_conv_noalloc.argtypes = [ctypes.POINTER(Desc), cf, cf, vp, vp, vp, ctypes.POINTER(Epilogue), ci, vp,
                                         vp]
     L.convgemm_conv_noalloc.restype = ci
@@ -387,6 +389,12 @@ def set_chunk_window(nbytes: int) -> None:
     """Test hook: bytes one batch chunk may address (0 restores 2 GiB); smaller values make small
     problems run the batch-chunking path."""
     lib().cg_set_chunk_window(int(nbytes))
+
+
+def set_dgrad_forward(on: bool) -> None:
+    """Test hook: backward data of unit-stride NHWC problems runs as the forward convolution of dY
+    with the kernel turned by 180 degrees (True, the default) or on the residue-class kernel."""
+    lib().cg_set_dgrad_forward(1 if on else 0)
 
 
 def quantize(src, dst, dtype, stream=None) -> float:

@@ -158,16 +158,17 @@ int cg_launch_scale(float *out, long long n, float beta, void *stream);
 /* dst = src with its three dimensions [d0][d1][d2] permuted:
  *   perm 0: [d0][d1][d2] -> [d2][d1][d0]
  *   perm 1: [d0][d1][d2] -> [d0][d2][d1]
+ *   perm 2: [d0][d1][d2] -> [d0][d1 - 1 - i][d2] (d1 reversed)
  * esize: element size in bytes (4, 2 or 1). */
 /* FP32 -> BF16 (elem CG_ELEM_BF16) or FP8 e4m3fn with a per-tensor amax
  * scale (CG_ELEM_FP8; `scratch`: 8 device bytes, the scale lands at +4). */
 int cg_launch_quantize(const float *src, void *dst, long long n, int elem, void *scratch, void *stream);
 int cg_launch_permute3(const void *src, void *dst, int d0, int d1, int d2, int perm, int esize, void *stream);
 
-/* Element-wise jobs of a batch run in one launch: weight repacks (kind 0, 1:
- * cg_launch_permute3's perm 0, 1 of [d0][d1][d2] on esize-byte elements) and
- * beta scaling of n FP32 outputs (kind 2: dst[i] = beta * dst[i], 0 if beta
- * == 0).  Job j owns items [start[j], start[j + 1]), cg_ew_items() of them
+/* Element-wise jobs of a batch run in one launch: weight repacks (kind 0, 1,
+ * 3: cg_launch_permute3's perm 0, 1, 2 of [d0][d1][d2] on esize-byte
+ * elements, n of them) and beta scaling of n FP32 outputs (kind 2: dst[i] =
+ * beta * dst[i], 0 if beta == 0).  Job j owns items [start[j], start[j + 1]), cg_ew_items() of them
  * (32x32 transpose tiles, or blocks of CG_EW_BLOCK elements).  jobs and start
  * are device arrays. */
 typedef struct {

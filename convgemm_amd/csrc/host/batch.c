@@ -102,7 +102,9 @@ static convgemm_status_t plan_one(const convgemm_problem_t *q, convgemm_dtype_t 
     if (chunk < d->b)
         return errf(CONVGEMM_ERR_UNSUPPORTED, "a batch problem exceeds the 2 GiB addressing window");
     const int es = cg_esize(elem);
-    const long long P = (long long)d->b * kp.Ho * kp.Wo;
+    /* output pixels of the convolution (kp may describe a backward pass in its forward form) */
+    const long long P = (long long)d->b * convgemm_out_size(d->h, d->vpad, d->vdil, d->kh, d->vstride) *
+                        convgemm_out_size(d->w, d->hpad, d->hdil, d->kw, d->hstride);
     const long long K = (long long)d->c * d->kh * d->kw;
     long long gm, gn, gk, out_elems;
     if (d->op == CONVGEMM_FWD) {
@@ -361,7 +363,7 @@ convgemm_status_t convgemm_batch_create(const convgemm_problem_t *problems, int 
             cg_ewjob *j = &hjobs[nj++];
             j->src = bp[i].wsrc;
             j->dst = wptr;
-            j->kind = bp[i].perm ? 1 : 0;
+            j->kind = bp[i].perm == 2 ? 3 : bp[i].perm; /* 0, 1: transposes; 3: reversed taps */
             j->d0 = bp[i].d0; j->d1 = bp[i].d1; j->d2 = bp[i].d2;
             j->n = (long long)bp[i].d0 * bp[i].d1 * bp[i].d2;
             bp[i].kp.Bm = wptr;

@@ -84,4 +84,8 @@ convgemm_status_t cg_growbuf_get(cg_growbuf *gb, size_t need, void *stream, void
  * values make small problems run the batch-chunking path. */
 void cg_set_chunk_window(long long bytes);
 
+/* Test hook: run backward data of unit-stride NHWC problems in its forward
+ * form (1, the default) or on the residue-class kernel (0). */
+void cg_set_dgrad_forward(int on);
+
 #endif

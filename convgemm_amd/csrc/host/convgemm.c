@@ -55,6 +55,8 @@ static int g_fail_allocs = 0;
 /* bytes one batch chunk may address (test hook; 0: CG_WINDOW) */
 static long long g_chunk_window = 0;
 void cg_set_chunk_window(long long bytes) { g_chunk_window = bytes > 0 && bytes < CG_WINDOW ? bytes : 0; }
+static int g_dgrad_fwd = 1;
+void cg_set_dgrad_forward(int on) { g_dgrad_fwd = on != 0; }
 
 static convgemm_status_t set_error(convgemm_status_t st, const char *fmt, ...) {
     t_last_status = st;
@@ -494,10 +496,41 @@ typedef struct {
 
 /* Weight order the kernels read, and the repack that produces it
  * (arguments of cg_launch_permute3); returns 0 when none is needed. */
+/* Backward data of an NHWC convolution with unit strides is the forward
+ * convolution of dY with the kernel turned by 180 degrees (taps in reverse
+ * order, roles of c and kn swapped): dX[b][ih][iw][c] = sum over (kx, ky, kn)
+ * of dY[b][ih + kx*vd - p'v][iw + ky*hd - p'h][kn] * W[c][KH-1-kx][KW-1-ky][kn]
+ * with p' = (K - 1) * dil - pad, the gather of src/convGemm.c:86-116 read as
+ * a convolution.  It then runs on the forward kernels (pipelined and halo
+ * tiles).  A negative p' (padding beyond the kernel's reach) keeps the
+ * residue-class kernel. */
+static int dgrad_as_forward(const convgemm_desc_t *d) {
+    return g_dgrad_fwd && d->op == CONVGEMM_DGRAD && d->layout == CONVGEMM_NHWC && d->vstride == 1 &&
+           d->hstride == 1 && (d->kh - 1) * d->vdil >= d->vpad && (d->kw - 1) * d->hdil >= d->hpad;
+}
+
+/* Rewrite a prepared backward-data problem (kp->Ho, Wo: dY's extent) into its forward form. */
+static void forward_form(const convgemm_desc_t *d, cg_kparams *kp) {
+    kp->op = CG_FWD_NHWC;
+    kp->H = kp->Ho;
+    kp->W = kp->Wo;
+    kp->C = d->kn;
+    kp->KN = d->c;
+    kp->vp = (d->kh - 1) * d->vdil - d->vpad;
+    kp->hp = (d->kw - 1) * d->hdil - d->hpad;
+    kp->Ho = d->h;
+    kp->Wo = d->w;
+}
+
 static int weight_repack(const convgemm_desc_t *d, int elem, int *d0, int *d1, int *d2, int *perm) {
     int taps = d->kh * d->kw;
     int K = d->c * taps;
     if (d->op == CONVGEMM_WGRAD) return 0;
+    if (dgrad_as_forward(d)) { /* [c][tap][kn] -> [c][taps-1-tap][kn]: the forward form's [kn'][tap][c'] */
+        if (taps == 1) return 0;
+        *d0 = d->c; *d1 = taps; *d2 = d->kn; *perm = 2;
+        return 1;
+    }
     if (d->op == CONVGEMM_FWD) {
         if (d->layout == CONVGEMM_NHWC) { /* [c][tap][kn] -> [kn][tap][c] */
             *d0 = d->c; *d1 = taps; *d2 = d->kn; *perm = 0;
@@ -585,6 +618,7 @@ convgemm_status_t cg_prepare(const convgemm_desc_t *d, convgemm_dtype_t dtype, c
     const int repack = weight_repack(d, elem, &rd0, &rd1, &rd2, &rperm);
     /* an internal workspace is 256-byte aligned, so only caller buffers can break vec */
     kp->vec = vec_ok(elem, d, x, repack ? wsw : in);
+    if (dgrad_as_forward(d)) forward_form(d, kp);
     long long img_a, img_o, img_b;
     image_bytes(d, ho, wo, es, &img_a, &img_o, &img_b);
     long long img_max = img_a > img_o ? img_a : img_o;
@@ -594,7 +628,7 @@ convgemm_status_t cg_prepare(const convgemm_desc_t *d, convgemm_dtype_t dtype, c
     if (chunk > d->b) chunk = d->b;
     if (chunk < 1) chunk = 1;
     kp->B = chunk;
-    kp->P = (long long)chunk * ho * wo;
+    kp->P = (long long)chunk * kp->Ho * kp->Wo;
     *elem_out = elem;
     *chunk_out = chunk;
     return CONVGEMM_SUCCESS;
@@ -672,7 +706,8 @@ static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, 
     convgemm_status_t st = cg_prepare(d, dtype, in, x, ws->prepacked ? in : ws->ws_w, &kp, &elem, &chunk);
     if (st != CONVGEMM_SUCCESS) return st;
     const int es = esize_of(elem);
-    const int ho = kp.Ho, wo = kp.Wo;
+    int ho, wo; /* the convolution's output extent (kp's may be that of a forward form) */
+    validate(d, &ho, &wo);
 
     const long long P = (long long)d->b * ho * wo;
     const long long K = (long long)d->c * d->kh * d->kw;
@@ -776,7 +811,7 @@ static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, 
         cg_kparams c = kp;
         int nb = d->b - b0 < chunk ? d->b - b0 : chunk;
         c.B = nb;
-        c.P = (long long)nb * ho * wo;
+        c.P = (long long)nb * kp.Ho * kp.Wo;
         cg_fill_tiles(&c, &t, kp.splits);
         if (kp.sk) { /* this chunk's own schedule, within the pieces the workspace holds */
             c.sk = cg_sk_split(elem, cfg, &c, &c.sk_dp);
@@ -950,8 +985,10 @@ size_t convgemm_workspace_size(const convgemm_desc_t *d, convgemm_dtype_t dtype)
     kp.op = opkind(d);
     kp.B = d->b; kp.H = d->h; kp.W = d->w; kp.C = d->c; kp.KN = d->kn; kp.KH = d->kh; kp.KW = d->kw;
     kp.vs = d->vstride; kp.hs = d->hstride;
+    kp.vp = d->vpad; kp.hp = d->hpad; kp.vd = d->vdil; kp.hd = d->hdil;
     kp.Ho = ho; kp.Wo = wo; kp.taps = d->kh * d->kw;
-    kp.P = (long long)d->b * ho * wo;
+    if (dgrad_as_forward(d)) forward_form(d, &kp);
+    kp.P = (long long)d->b * kp.Ho * kp.Wo;
     size_t parts = 0;
     int n = cg_num_tilecfgs(elem);
     for (int cfg = 0; cfg < n; cfg++) {

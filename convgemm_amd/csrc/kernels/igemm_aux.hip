@@ -280,7 +280,7 @@ __host__ __device__ inline EwGeom ew_geom(const cg_ewjob &j) {
 }
 
 extern "C" long long cg_ew_items(const cg_ewjob *j) {
-    if (j->kind == 2) return (j->n + CG_EW_BLOCK - 1) / CG_EW_BLOCK;
+    if (j->kind >= 2) return (j->n + CG_EW_BLOCK - 1) / CG_EW_BLOCK;
     const EwGeom g = ew_geom(*j);
     return (long long)g.Z * ((g.R + 31) / 32) * ((g.C + 31) / 32);
 }
@@ -306,6 +306,18 @@ __global__ void __launch_bounds__(256) ew_grouped_kernel(const cg_ewjob *__restr
             for (long long i = i0 + threadIdx.x; i < i1; i += 256) out[i] = j.beta == 0.0f ? 0.0f : j.beta * out[i];
             continue;
         }
+        if (j.kind == 3) {  // [d0][d1][d2] -> [d0][d1 - 1 - t][d2]
+            const E *src = (const E *)j.src;
+            E *dst = (E *)j.dst;
+            const int plane = j.d1 * j.d2;
+            const int i0 = local * CG_EW_BLOCK, i1 = (int)min((long long)i0 + CG_EW_BLOCK, j.n);
+            for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) {
+                const int a = i / plane, rem = i - a * plane;
+                const int t = rem / j.d2, k = rem - t * j.d2;
+                dst[i] = src[a * plane + (j.d1 - 1 - t) * j.d2 + k];
+            }
+            continue;
+        }
         const EwGeom g = ew_geom(j);
         const int tr = (g.R + 31) / 32, tcn = (g.C + 31) / 32;
         const int z = local / (tr * tcn), rem = local - z * (tr * tcn);
@@ -322,6 +334,21 @@ __global__ void __launch_bounds__(256) ew_grouped_kernel(const cg_ewjob *__restr
             const int c = c0 + i, r = r0 + tx;
             if (r < g.R && c < g.C) dst[(long long)c * g.tc + r] = t[tx][i];
         }
+    }
+}
+
+// [d0][d1][d2] -> [d0][d1 - 1 - t][d2] (a kernel's taps reversed): element i
+// of the output reads its mirror row of the input.
+template <typename E>
+__global__ void reverse_mid_kernel(const E *__restrict__ src, E *__restrict__ dst, int d1, int d2, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long plane = (long long)d1 * d2;
+    for (; i < n; i += stride) {
+        const long long a = i / plane;
+        const int rem = (int)(i - a * plane);
+        const int t = rem / d2, k = rem - t * d2;
+        dst[i] = src[a * plane + (long long)(d1 - 1 - t) * d2 + k];
     }
 }
 
@@ -507,6 +534,22 @@ static int launch_transpose(const void *src, void *dst, int R, int Cc, int batch
 extern "C" int cg_launch_permute3(const void *src, void *dst, int d0, int d1, int d2, int perm, int esize,
                                   void *stream) {
     if (d0 <= 0 || d1 <= 0 || d2 <= 0) return 0;
+    if (perm == 2) {  // [d0][d1][d2] -> [d0][d1 reversed][d2]: rows of d2 elements move whole
+        const long long n = (long long)d0 * d1 * d2;
+        long long blocks = (n + 255) / 256;
+        if (blocks > 8192) blocks = 8192;
+        hipStream_t st = (hipStream_t)stream;
+        switch (esize) {
+        case 4: hipLaunchKernelGGL(reverse_mid_kernel<float>, dim3((unsigned)blocks), dim3(256), 0, st,
+                                   (const float *)src, (float *)dst, d1, d2, n); break;
+        case 2: hipLaunchKernelGGL(reverse_mid_kernel<uint16_t>, dim3((unsigned)blocks), dim3(256), 0, st,
+                                   (const uint16_t *)src, (uint16_t *)dst, d1, d2, n); break;
+        case 1: hipLaunchKernelGGL(reverse_mid_kernel<uint8_t>, dim3((unsigned)blocks), dim3(256), 0, st,
+                                   (const uint8_t *)src, (uint8_t *)dst, d1, d2, n); break;
+        default: return (int)hipErrorInvalidValue;
+        }
+        return (int)hipGetLastError();
+    }
     int R, Cc, batch;
     long long sz, sr, tz, tc;
     if (perm == 0) {  // [d0][d1][d2] -> [d2][d1][d0]: batch over d1
