@@ -437,6 +437,105 @@ def explicit_vs_implicit(torch, cg, stream, reps=10, batch=32):
     return res
 
 
+REF_CATALOGUE_MEDIANS_GFS = {  # FP32, batch 64, host "volta" (plots/test.out, test_back.out, test_trans.out)
+    "fwd_nhwc": 906.0, "fwd_nchw": 437.0, "dgrad_nhwc": 366.0, "dgrad_nchw": 159.0,
+    "wgrad_nhwc": 296.0, "wgrad_nchw": 293.0}
+
+
+def catalogue_shapes(batch=64):
+    """The reference's CNN layer catalogue (tests/test.txt, committed as
+    tests/golden/cnn_layer_shapes.txt; columns h w c b kh kw kn stride padding):
+    the distinct rows at `batch`, as tests/test.pl feeds them to the drivers --
+    220 shapes at batch 64, the set plots/test*.out hold."""
+    seen, rows = set(), []
+    for line in open(os.path.join(ROOT, "tests", "golden", "cnn_layer_shapes.txt")):
+        if line.startswith("#") or not line.strip():
+            continue
+        h, w, c, b, kh, kw, kn, st, pad = map(int, line.split())
+        key = (h, w, c, b, kh, kw, kn, st, pad)
+        if b != batch or key in seen:
+            continue
+        seen.add(key)
+        rows.append((b, h, w, c, kn, kh, kw, pad, st))
+    return sorted(rows, key=lambda s: s[1] * s[2] * s[3] * s[0] * s[5] * s[6] * s[4])
+
+
+def catalogue_sweep(torch, cg, stream, reps=3, batch=64):
+    """FP32 throughput over the reference's whole shape catalogue at its batch
+    (220 layers, batch 64), the set its published logs hold: forward with the
+    fused bias + batch-norm + ReLU epilogue, weight gradient and backward data
+    (dX += ...), NHWC and NCHW, each planned by the default planner (the
+    learned cost model; no autotuning), timed with HIP events over `reps`
+    back-to-back calls after one warm-up call.  GF/s = 2*m*n*k / t as in
+    plots/convgemm.plt:56-57.  Reported: the median and quartiles over the
+    catalogue per operation beside the reference's own medians on its CPU,
+    and, as a size-independent check of every shape, the adjoint identity
+    <conv(X,W), dY> = <X, dgrad(W,dY)> = <W, wgrad(X,dY)> (plain calls, FP64
+    dot products on the GPU)."""
+    shapes = catalogue_shapes(batch)
+    gfs = {k: [] for k in REF_CATALOGUE_MEDIANS_GFS}
+    worst_gap = 0.0
+    worst = None
+    t_start = time.perf_counter()
+    for si, spec in enumerate(shapes):
+        b, h, w, c, kn, kh, kw, pad, st = spec
+        ho, wo = out_hw(h, pad, kh, st), out_hw(w, pad, kw, st)
+        if ho <= 0 or wo <= 0:
+            continue
+        flops = layer_flops(spec)
+        for layout, lname in ((0, "nhwc"), (1, "nchw")):
+            d = make_inputs(torch, layout, spec, 0xC0FFEE + 9000 + 2 * si + layout)
+            dY = torch.randn(d["Y"].shape, device="cuda")
+            dX = torch.zeros(d["X"].shape, device="cuda")
+            dW = torch.empty(d["W"].shape, device="cuda")
+            fdesc = cg.make_desc(cg.FWD, layout, b, h, w, c, kn, kh, kw, pad, pad, st, st)
+            wdesc = cg.make_desc(cg.WGRAD, layout, b, h, w, c, kn, kh, kw, pad, pad, st, st)
+            ddesc = cg.make_desc(cg.DGRAD, layout, b, h, w, c, kn, kh, kw, pad, pad, st, st)
+            calls = {
+                "fwd": lambda: cg.conv_ex(fdesc, d["W"], d["X"], d["Y"], bias=d["bias"], mean=d["mean"],
+                                          inv_std=d["inv_std"], gamma=d["gamma"], bnbeta=d["bnbeta"], relu=True,
+                                          stream=stream),
+                "wgrad": lambda: cg.conv_ex(wdesc, dY, d["X"], dW, stream=stream),
+                "dgrad": lambda: cg.conv_ex(ddesc, d["W"], dY, dX, beta=1.0, stream=stream),
+            }
+            for op, fn in calls.items():
+                fn()
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(reps):
+                    fn()
+                e1.record(stream)
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / reps
+                gfs[f"{op}_{lname}"].append(flops / (ms * 1e-3) / 1e9)
+            # adjoint identity on plain calls (no epilogue, dX overwritten with alpha=1, beta=0)
+            cg.conv_ex(fdesc, d["W"], d["X"], d["Y"], stream=stream)
+            cg.conv_ex(wdesc, dY, d["X"], dW, stream=stream)
+            cg.conv_ex(ddesc, d["W"], dY, dX, beta=0.0, stream=stream)
+            torch.cuda.synchronize()
+            s_y = float((d["Y"].double() * dY.double()).sum())
+            s_x = float((d["X"].double() * dX.double()).sum())
+            s_w = float((d["W"].double() * dW.double()).sum())
+            scale = max(float(d["Y"].double().norm() * dY.double().norm()), 1e-30)
+            gap = max(abs(s_y - s_x), abs(s_y - s_w)) / scale
+            if gap > worst_gap:
+                worst_gap, worst = gap, list(spec)
+            del d, dY, dX, dW
+    out = {"shapes": len(shapes), "batch": batch, "dtype": "f32", "planner": "learned cost model (default)",
+           "reps": reps, "wall_s": round(time.perf_counter() - t_start, 1)}
+    for k, v in gfs.items():
+        a = np.array(v)
+        med = float(np.median(a))
+        out[k] = {"median_gflops": round(med, 1), "q1": round(float(np.percentile(a, 25)), 1),
+                  "q3": round(float(np.percentile(a, 75)), 1), "max": round(float(a.max()), 1),
+                  "reference_cpu_median_gflops": REF_CATALOGUE_MEDIANS_GFS[k],
+                  "ratio_vs_reference_median": round(med / REF_CATALOGUE_MEDIANS_GFS[k], 1)}
+    out["adjoint_identity_max_rel_gap"] = worst_gap
+    out["adjoint_identity_worst_shape"] = worst
+    return out
+
+
 def _cpu_problems(batch=32):
     """The bench step's ten forward convolutions at `batch` on the host (FP32, seeded)."""
     from oracle import Shape
@@ -585,6 +684,10 @@ def main(argv=None):
     ap.add_argument("--backward-only", action="store_true",
                     help="only run the backward sweep and print it (for profiling its kernels)")
     ap.add_argument("--no-ablation", action="store_true", help="skip the explicit-vs-implicit comparison")
+    ap.add_argument("--no-catalogue", action="store_true",
+                    help="skip the FP32 sweep over the reference's 220-layer shape catalogue at batch 64")
+    ap.add_argument("--catalogue-only", action="store_true",
+                    help="only run the shape-catalogue sweep and print it")
     ap.add_argument("--cpu-worker", type=int, default=0, metavar="REPS",
                     help="internal: run one CPU-baseline process with REPS timed steps and exit")
     ap.add_argument("--only", default="", help="comma-separated layer names (e.g. L2_3x3_64to64_56_NHWC) "
@@ -616,6 +719,9 @@ def main(argv=None):
         if os.path.exists(SHIPPED_BWD_DB):
             cg.tune_load(SHIPPED_BWD_DB)
         plans = "shipped tuning database (MI355X autotuner timings)"
+    if a.catalogue_only:
+        print(json.dumps({"catalogue_sweep": catalogue_sweep(torch, cg, stream)}), flush=True)
+        return 0
     if a.backward_only:
         res = {"backward_sweep": backward_sweep(torch, cg, stream), "plans": plans}
         if a.tune_db:
@@ -778,6 +884,8 @@ def main(argv=None):
         out["batch_sweep"] = batch_sweep(torch, cg, stream)
     if not a.no_ablation:
         out["explicit_vs_implicit"] = explicit_vs_implicit(torch, cg, stream)
+    if not a.no_catalogue:
+        out["catalogue_sweep"] = catalogue_sweep(torch, cg, stream)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
