@@ -61,7 +61,7 @@ EXPORTED_SYMBOLS = (
 
 PLAN_ANALYTIC, PLAN_TUNED, PLAN_LEARNED, PLAN_MODEL = 0, 1, 2, 3
 # CONVGEMM_MAX_CANDIDATES of convgemm_tune.h: candidate plans of one problem at most
-MAX_CANDIDATES = 2048
+MAX_CANDIDATES = 4096
 
 
 class ConvGemmError(RuntimeError):

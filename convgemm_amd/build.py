@@ -75,6 +75,8 @@ def slices():
         out.append((tn, ct, 0, 1))
         if tn == "f32":
             out.append((tn, ct, 1, 1))
+        if tn != "fp8":
+            out.append((tn, ct, 2, 1))
     # the pipelined slices are the largest translation units: start them first
     return sorted(out, key=lambda s: (-s[3], s[2]))
 

@@ -351,11 +351,18 @@ int cg_tune_get(const convgemm_desc_t *desc, convgemm_dtype_t dtype, int *cfgs, 
 /* ------------------------------------------------------------ persistence */
 
 /* Text format, one problem per line:
- *   16 key ints, candidate count n, then n triples "cfg splits microseconds" */
+ *   16 key ints, candidate count n, then n triples "cfg splits microseconds"
+ * after a header that records the tile shapes of each operand type.  A
+ * configuration is tile shape base + raster order r * (number of shapes), and
+ * tables only grow by appending shapes, so the ids of a database written with
+ * fewer shapes are renumbered on load: base + r * (shapes now). */
+#define TILE_SHAPES_TAG "# tile shapes (f32 bf16 fp8):"
 convgemm_status_t convgemm_tune_save(const char *path) {
     FILE *f = fopen(path, "w");
     if (!f) return cg_set_error(CONVGEMM_ERR_INVALID_VALUE, "cannot open tuning database for writing");
     fprintf(f, "# convgemm tuning database v1: elem op vec B H W C KN KH KW vp hp vs hs vd hd | n (cfg splits us)*n\n");
+    fprintf(f, TILE_SHAPES_TAG " %d %d %d\n", cg_num_tile_shapes(CG_ELEM_F32), cg_num_tile_shapes(CG_ELEM_BF16),
+            cg_num_tile_shapes(CG_ELEM_FP8));
     pthread_mutex_lock(&t_lock);
     for (int i = 0; i < t_cap; i++) {
         tentry *e = &t_tab[i];
@@ -382,7 +389,13 @@ convgemm_status_t convgemm_tune_load(const char *path) {
         return cg_set_error(CONVGEMM_ERR_ALLOC, "tuning database: out of host memory");
     }
     int loaded = 0;
+    int old_ns[3] = {0, 0, 0}; /* tile shapes of the writer, 0: not recorded (ids taken as they are) */
     while (fgets(line, line_max, f)) {
+        if (strncmp(line, TILE_SHAPES_TAG, strlen(TILE_SHAPES_TAG)) == 0) {
+            if (sscanf(line + strlen(TILE_SHAPES_TAG), "%d %d %d", &old_ns[0], &old_ns[1], &old_ns[2]) != 3)
+                old_ns[0] = old_ns[1] = old_ns[2] = 0;
+            continue;
+        }
         if (line[0] == '#' || line[0] == '\n') continue;
         int k[KEYLEN], n, pos = 0, used;
         char *p = line;
@@ -400,6 +413,20 @@ convgemm_status_t convgemm_tune_load(const char *path) {
             p += used;
         }
         if (pos != n) continue;
+        const int e = k[0];
+        if (e >= 0 && e < 3 && old_ns[e] > 0) {
+            const int ns = cg_num_tile_shapes(e);
+            int kept = 0;
+            for (int j = 0; j < n; j++) {
+                const int base = cfg[j] % old_ns[e], r = cfg[j] / old_ns[e];
+                if (base >= ns || r >= CG_NUM_RASTERS) continue; /* a shape this build does not have */
+                cfg[kept] = base + r * ns;
+                spl[kept] = spl[j];
+                us[kept] = us[j];
+                kept++;
+            }
+            n = kept;
+        }
         pthread_mutex_lock(&t_lock);
         db_put(k, n, cfg, spl, us);
         pthread_mutex_unlock(&t_lock);

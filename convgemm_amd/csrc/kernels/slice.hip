@@ -10,6 +10,7 @@
 #include "lp_kernel.hpp"
 #include "lp_nchw_kernel.hpp"
 #include "ht_kernel.hpp"
+#include "lpw_kernel.hpp"
 #include "tile_tables.hpp"
 
 using namespace cg;
@@ -31,6 +32,8 @@ int launch_slice(int cfg, const cg_kparams *kp, const cg_group *g, int *occ, hip
                 return launch_lpn<d.BM, d.BN, d.WM, d.WN, d.NS>(kp, g, occ, st);
             else if constexpr (FAM == 1 && d.fam == FAM_HT && OP == CG_FWD_NHWC)
                 return launch_ht<T, d.BM, d.BN, d.WM, d.WN, d.NS, d.HS>(kp, g, occ, st);
+            else if constexpr (FAM == 1 && d.fam == FAM_LPW && OP == CG_WGRAD_NHWC && sizeof(T) != 1)
+                return launch_lpw<T, d.BM, d.BN, d.WM, d.WN, d.NS>(kp, g, occ, st);
             else
                 return (int)hipErrorInvalidValue;
         }

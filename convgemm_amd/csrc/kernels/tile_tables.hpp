@@ -9,7 +9,9 @@
 //   fam      kernel family: GEN = igemm_tile (every operation), LP = the
 //            pipelined forward NHWC kernel, LPN = the pipelined forward NCHW
 //            kernel (FP32), HT = the halo-tile forward NHWC kernel
-//            (ht_kernel.hpp; kernels larger than 1x1 whose halo fits HS)
+//            (ht_kernel.hpp; kernels larger than 1x1 whose halo fits HS),
+//            LPW = the pipelined weight-gradient NHWC kernel (lpw_kernel.hpp;
+//            BK is its K step in pixels)
 //   nv       the general kernel also has element-load (non-16-byte)
 //            instantiations of this tile
 //   HS       halo slots (128 bytes each) of an HT tile
@@ -20,7 +22,7 @@
 
 namespace cg {
 
-enum TileFam { FAM_GEN = 0, FAM_LP = 1, FAM_LPN = 2, FAM_HT = 3 };
+enum TileFam { FAM_GEN = 0, FAM_LP = 1, FAM_LPN = 2, FAM_HT = 3, FAM_LPW = 4 };
 
 struct TileDef {
     int BM, BN, BK, WM, WN, NS;
@@ -71,6 +73,12 @@ template <> struct Tiles<float> {
         {64, 128, 32, 2, 2, 4, FAM_HT, false, 128},   // 35 halo tile
         {64, 64, 32, 2, 2, 4, FAM_HT, false, 288},    // 36 halo tile
         {256, 128, 32, 4, 2, 3, FAM_HT, false, 384},  // 37 halo tile
+        {128, 128, 32, 2, 2, 3, FAM_LPW, false},  // 38 pipelined weight gradient
+        {128, 128, 32, 2, 2, 2, FAM_LPW, false},  // 39
+        {128, 64, 32, 2, 2, 3, FAM_LPW, false},   // 40
+        {64, 128, 32, 2, 2, 3, FAM_LPW, false},   // 41
+        {64, 64, 32, 2, 2, 4, FAM_LPW, false},    // 42
+        {256, 128, 32, 4, 2, 2, FAM_LPW, false},  // 43 8 waves
     };
 };
 
@@ -113,6 +121,14 @@ template <> struct Tiles<uint16_t> {
         {64, 128, 64, 2, 2, 4, FAM_HT, false, 128},   // 34 halo tile
         {64, 64, 64, 2, 2, 4, FAM_HT, false, 288},    // 35 halo tile
         {256, 128, 64, 4, 2, 3, FAM_HT, false, 384},  // 36 halo tile
+        {128, 128, 64, 2, 2, 3, FAM_LPW, false},  // 37 pipelined weight gradient
+        {128, 128, 64, 2, 2, 2, FAM_LPW, false},  // 38
+        {128, 64, 64, 2, 2, 3, FAM_LPW, false},   // 39
+        {64, 128, 64, 2, 2, 3, FAM_LPW, false},   // 40
+        {64, 64, 64, 2, 2, 4, FAM_LPW, false},    // 41
+        {256, 128, 64, 4, 2, 2, FAM_LPW, false},  // 42 8 waves
+        {128, 256, 64, 2, 4, 2, FAM_LPW, false},  // 43 8 waves
+        {128, 64, 64, 2, 2, 2, FAM_LPW, false},   // 44
     };
 };
 

@@ -7,7 +7,8 @@
 // Slices are referenced weakly: the build decides which (type, operation,
 // family) slices exist, and a slice that was not built resolves to null here
 // and reports hipErrorInvalidValue (FP8 has forward slices only; the
-// pipelined family exists for the forward operations).
+// pipelined family exists for the forward operations and the NHWC weight
+// gradient).
 #include "igemm_kernel.hpp"
 #include "tile_tables.hpp"
 
@@ -17,7 +18,8 @@ using namespace cg;
     extern "C" __attribute__((weak)) int cg_slice_##tn##_##op##_##fam(int, const cg_kparams *, const cg_group *, int *, void *);
 #define CG_SLICE_DECL_T(tn)                                                                                        \
     CG_SLICE_DECL(tn, 0, 0) CG_SLICE_DECL(tn, 1, 0) CG_SLICE_DECL(tn, 2, 0) CG_SLICE_DECL(tn, 3, 0)               \
-    CG_SLICE_DECL(tn, 4, 0) CG_SLICE_DECL(tn, 5, 0) CG_SLICE_DECL(tn, 0, 1) CG_SLICE_DECL(tn, 1, 1)
+    CG_SLICE_DECL(tn, 4, 0) CG_SLICE_DECL(tn, 5, 0) CG_SLICE_DECL(tn, 0, 1) CG_SLICE_DECL(tn, 1, 1)               \
+    CG_SLICE_DECL(tn, 2, 1)
 CG_SLICE_DECL_T(f32)
 CG_SLICE_DECL_T(bf16)
 CG_SLICE_DECL_T(fp8)
@@ -26,7 +28,7 @@ namespace {
 
 #define CG_SLICE_ROW(tn)                                                                                           \
     {{cg_slice_##tn##_0_0, cg_slice_##tn##_0_1}, {cg_slice_##tn##_1_0, cg_slice_##tn##_1_1},                       \
-     {cg_slice_##tn##_2_0, nullptr}, {cg_slice_##tn##_3_0, nullptr},                                               \
+     {cg_slice_##tn##_2_0, cg_slice_##tn##_2_1}, {cg_slice_##tn##_3_0, nullptr},                                               \
      {cg_slice_##tn##_4_0, nullptr}, {cg_slice_##tn##_5_0, nullptr}}
 // [elem][op][family: 0 general, 1 pipelined]
 const slice_fn kSlices[3][CG_NUM_OPKINDS][2] = {CG_SLICE_ROW(f32), CG_SLICE_ROW(bf16), CG_SLICE_ROW(fp8)};
@@ -58,6 +60,7 @@ const TileDef *def_of(int elem, int cfg, int *base = nullptr, int *raster = null
 template <typename T> int supports_t(const TileDef &d, int op, int vec) {
     if (d.fam == FAM_LP || d.fam == FAM_HT) return op == CG_FWD_NHWC && vec;
     if (d.fam == FAM_LPN) return op == CG_FWD_NCHW && vec;
+    if (d.fam == FAM_LPW) return op == CG_WGRAD_NHWC && vec && sizeof(T) != 1;
     if (!op_enabled<T>(op)) return 0;
     if (!vec && vec_relevant<T>(op) && !d.nv) return 0;
     // tiles of 256x128 and up spill registers in the backward-data gather

@@ -71,7 +71,7 @@ convgemm_status_t convgemm_costmodel_save(const char *path);
 convgemm_status_t convgemm_costmodel_load(const char *path);
 /* Upper bound on the candidate plans (configuration, split) of one problem:
  * arrays of this length hold every candidate of any problem. */
-#define CONVGEMM_MAX_CANDIDATES 2048
+#define CONVGEMM_MAX_CANDIDATES 4096
 
 /* Predicted time (microseconds) of every candidate of a problem; out[i] is
  * for (cfgs[i], splits[i]).  n_out: capacity in, candidates written out. */

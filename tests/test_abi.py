@@ -256,6 +256,33 @@ def test_tuning_database_holds_full_candidate_lists(tmp_path):
     assert all(abs(g - u) <= 1e-3 * u for (_, _, g), u in zip(got, us))
 
 
+def test_tuning_database_renumbers_configurations(tmp_path):
+    """Configuration ids are tile shape + raster order x (number of shapes); a database
+    written by a build with fewer tile shapes (its header records them) is renumbered on
+    load, and a save records this build's counts so that a reload is the identity."""
+    d = cg.make_desc(cg.FWD, cg.NHWC, 2, 8, 8, 16, 16, 3, 3, 1, 1, 1, 1)
+    key = "0 0 1 2 8 8 16 16 3 3 1 1 1 1 1 1"
+    ns = cg.num_tile_shapes(cg.F32)
+    old = ns - 6  # a writer with six fewer FP32 shapes
+    old_ids = [3, old + 5, 2 * old + 1]  # shape 3 order 0, shape 5 order 1, shape 1 order 2
+    line = key + " 3 " + " ".join(f"{c} 1 {10.0 + i:.3f}" for i, c in enumerate(old_ids))
+    path = tmp_path / "old.tunedb"
+    path.write_text(f"# header\n# tile shapes (f32 bf16 fp8): {old} 40 30\n{line}\n")
+    cg.tune_clear()
+    try:
+        cg.tune_load(str(path))
+        got = [c for c, _, _ in cg.tune_get(d, cg.F32)]
+        assert got == [3, ns + 5, 2 * ns + 1], got
+        again = tmp_path / "again.tunedb"
+        cg.tune_save(str(again))
+        assert f"# tile shapes (f32 bf16 fp8): {ns} {cg.num_tile_shapes(cg.BF16)} " in again.read_text()
+        cg.tune_clear()
+        cg.tune_load(str(again))
+        assert [c for c, _, _ in cg.tune_get(d, cg.F32)] == got
+    finally:
+        cg.tune_clear()
+
+
 def _taps_brute(r, pad, k, dil, s):
     # input rows i = r + s*t receive kernel tap kx when i + pad - kx*dil is a multiple of s
     return sum(1 for kx in range(k) if (r + pad - kx * dil) % s == 0)
