@@ -1,9 +1,10 @@
 """Batched grouped launches and hipGraph replay (include/convgemm_batch.h).
 
 A batch runs the same kernels with the same plans as one convgemm_conv_ex
-call per problem, so every output must be BIT-identical to the
-per-problem call: the grouped kernel only changes which workgroup computes
-which tile, never the arithmetic or its order.  Problems mix all six
+call per problem (a stream-K plan runs as its configuration's whole tiles),
+so every output must be BIT-identical to the per-problem call of that plan:
+the grouped kernel only changes which workgroup computes which tile, never
+the arithmetic or its order.  Problems mix all six
 operations, epilogues, alpha/beta, split-K, stride classes, BF16 and FP8.
 Graph batches are replayed several times with eager work (a fill, a
 conv_ex call) and synchronizations between replays, and every replay is
@@ -72,9 +73,20 @@ class Case:
                           scales=self.scales, **self.epi)
 
     def reference(self):
+        """The per-call result of the plan a batch runs: the call's own plan, except that a
+        stream-K plan runs in a batch as its configuration's whole tiles (split-K 1)."""
         ref = self.out0.clone()
         cg.conv_ex(self.desc, self.in_, self.x, ref, alpha=self.alpha, beta=self.beta, dtype=self.dtype,
                    scales=self.scales, **self.epi)
+        cfg, splits = cg.last_plan()[:2]
+        if splits == 0:  # only the planner picks stream-K here (no test forces splits 0 on a batch)
+            ref.copy_(self.out0)
+            cg.force_config(cfg, 1)
+            try:
+                cg.conv_ex(self.desc, self.in_, self.x, ref, alpha=self.alpha, beta=self.beta, dtype=self.dtype,
+                           scales=self.scales, **self.epi)
+            finally:
+                cg.force_config(-1, 0)
         torch.cuda.synchronize()
         return ref
 

@@ -18,7 +18,9 @@ max-normwise (max|got - ref| / max|ref|):
   sees: BF16 <= 1e-2 (the target's bar); FP8 <= 8e-2 (e4m3 keeps 3 mantissa
   bits, a relative rounding error up to 2^-4 per operand).
 The batch is also checked bit-identical, problem by problem, to one
-convgemm_conv_ex call per problem with the same plans.
+convgemm_conv_ex call per problem with the same plans (a stream-K plan runs
+in a batch as its configuration's whole tiles, so its reference call is made
+with split-K 1).
 """
 import os
 
@@ -146,6 +148,13 @@ def test_uneven_batch_256_learned_plans():
         for c in cases:
             r = torch.empty_like(c["y"])
             cg.conv_ex(c["d"], c["w"].view(torch.int16), c["x"].view(torch.int16), r, dtype=cg.BF16)
+            cfg, splits = cg.last_plan()[:2]
+            if splits == 0:  # a batch runs a stream-K plan as its configuration's whole tiles
+                cg.force_config(cfg, 1)
+                try:
+                    cg.conv_ex(c["d"], c["w"].view(torch.int16), c["x"].view(torch.int16), r, dtype=cg.BF16)
+                finally:
+                    cg.force_config(-1, 0)
             refs.append(r)
         torch.cuda.synchronize()
         print("graph batch", flush=True)
