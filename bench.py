@@ -10,10 +10,10 @@ Inputs are synthetic (X ~ N(0,1), He-scaled weights, epilogue vectors as in
 the workload description) and resident in HBM before timing starts.
 
 Plans: by default the tuning databases shipped in convgemm_amd/data (the
-autotuner's timings of these ten problems, and of their backward, on an
-MI355X: the plans the committed profiles were measured with) are loaded
-read-only; problems they lack, or every problem with --fresh-tune, are
-autotuned at setup (untimed).
+autotuner's timings of these ten problems, of their backward, and of the
+batch-64 precision sweep, on an MI355X: the plans the committed profiles were
+measured with) are loaded read-only; problems they lack, or every problem
+with --fresh-tune, are autotuned at setup (untimed).
 
 Prints one JSON line:  value = TFLOP/s over the timed steps, the roofline
 of the dominant kernel measured with HIP events on the launch stream, the
@@ -36,6 +36,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 SHIPPED_TUNE_DB = os.path.join(ROOT, "convgemm_amd", "data", "mi355x_fp32_sweep_b32.tunedb")
 # backward (weight gradient, backward data; FP32 and BF16) of the same layers at batch 32
 SHIPPED_BWD_DB = os.path.join(ROOT, "convgemm_amd", "data", "mi355x_backward_b32.tunedb")
+# forward NHWC of the same layers at batch 64 in FP32, BF16 and FP8 (the precision sweep)
+SHIPPED_PREC_DB = os.path.join(ROOT, "convgemm_amd", "data", "mi355x_precision_b64.tunedb")
 sys.path.insert(0, ROOT)
 
 METRIC = "conv TFLOP/s (% of roofline), BF16/FP8 vs FP32 plain at stated FP64 error"
@@ -135,7 +137,9 @@ def precision_sweep(torch, cg, stream, tune=True, reps=10, batch=64):
     (per-tensor amax scaling, FP32 accumulation), X = |N(0,1)| (post-ReLU),
     W ~ N(0, 0.02).  Per layer: TFLOP/s of the convolution kernels (operands
     already converted; the conversion is timed separately) and the error
-    against the FP64 oracle at sampled outputs."""
+    against the FP64 oracle at sampled outputs.  Plans: the loaded tuning
+    database, or autotuned here (untimed) for problems it lacks (all of them
+    with tune=True and no database)."""
     from oracle import Shape, sampled
     res = {}
     for li, (name, spec) in enumerate(LAYERS):
@@ -167,7 +171,7 @@ def precision_sweep(torch, cg, stream, tune=True, reps=10, batch=64):
                 torch.cuda.synchronize()
                 conv_ms = e0.elapsed_time(e1)
                 scales = (sw, sx) if dt == cg.FP8_E4M3 else None
-            if tune:
+            if tune and not cg.tune_get(desc, dt):
                 cg.autotune(desc, Wo, Xo, Y, dtype=dt, scales=scales, reps=3, stream=stream)
             for _ in range(3):
                 cg.conv_ex(desc, Wo, Xo, Y, dtype=dt, scales=scales, stream=stream)
@@ -681,6 +685,8 @@ def main(argv=None):
                     help="only time hipBLASLt (torch.matmul) on the layers' GEMM shapes, for calibration")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB per step (calibrates HBM byte counters)")
+    ap.add_argument("--precision-only", action="store_true",
+                    help="only run the BF16/FP8 precision sweep and print it (for profiling its kernels)")
     ap.add_argument("--backward-only", action="store_true",
                     help="only run the backward sweep and print it (for profiling its kernels)")
     ap.add_argument("--no-ablation", action="store_true", help="skip the explicit-vs-implicit comparison")
@@ -716,11 +722,18 @@ def main(argv=None):
         plans = f"tuning database {os.path.basename(a.tune_db)}, then autotuned"
     elif not a.tune_db and not a.fresh_tune and not a.no_tune and os.path.exists(SHIPPED_TUNE_DB):
         cg.tune_load(SHIPPED_TUNE_DB)
-        if os.path.exists(SHIPPED_BWD_DB):
-            cg.tune_load(SHIPPED_BWD_DB)
+        for db in (SHIPPED_BWD_DB, SHIPPED_PREC_DB):
+            if os.path.exists(db):
+                cg.tune_load(db)
         plans = "shipped tuning database (MI355X autotuner timings)"
     if a.catalogue_only:
         print(json.dumps({"catalogue_sweep": catalogue_sweep(torch, cg, stream)}), flush=True)
+        return 0
+    if a.precision_only:
+        res = {"precision_sweep": precision_sweep(torch, cg, stream, tune=not a.no_tune), "plans": plans}
+        if a.tune_db:
+            cg.tune_save(a.tune_db)
+        print(json.dumps(res), flush=True)
         return 0
     if a.backward_only:
         res = {"backward_sweep": backward_sweep(torch, cg, stream), "plans": plans}
