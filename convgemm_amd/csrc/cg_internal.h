@@ -74,6 +74,8 @@ typedef struct {
      * n / d == (n * m) >> s for 0 <= n < 2^31; d = Ho*Wo, Wo, C, hpitch. */
     unsigned fdm_howo, fdm_wo, fdm_c, fdm_hp;
     int fds_howo, fds_wo, fds_c, fds_hp;
+    int ffd;         /* 1: the forward form of a unit-stride NHWC backward-data problem (op is CG_FWD_NHWC;
+                        only the planner's problem key tells the two apart) */
 } cg_kparams;
 
 /* Tile configurations of the implicit-GEMM kernels: a tile shape and kernel

@@ -65,6 +65,14 @@ size_t cg_sk_bytes(int groups, const cg_tilecfg *t);
 /* The library stream of the drop-in calls (hipStream_t). */
 void *cg_get_stream(void);
 
+/* Backward data of NHWC convolutions with unit strides (and padding within
+ * the kernel's reach) runs as a forward convolution of dY with the kernel
+ * turned by 180 degrees: cg_forward_form rewrites such a prepared problem
+ * (Ho, Wo: dY's extent) in place and sets kp->ffd; cg_dgrad_forward_form_ok
+ * says whether a prepared backward-data problem takes that form. */
+void cg_forward_form(cg_kparams *kp);
+int cg_dgrad_forward_form_ok(const cg_kparams *kp);
+
 /* Is `stream` (hipStream_t) being captured into a graph? */
 int cg_stream_capturing(void *stream);
 

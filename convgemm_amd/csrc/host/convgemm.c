@@ -510,16 +510,23 @@ static int dgrad_as_forward(const convgemm_desc_t *d) {
 }
 
 /* Rewrite a prepared backward-data problem (kp->Ho, Wo: dY's extent) into its forward form. */
-static void forward_form(const convgemm_desc_t *d, cg_kparams *kp) {
+void cg_forward_form(cg_kparams *kp) {
+    const int c = kp->C, kn = kp->KN, h = kp->H, w = kp->W;
     kp->op = CG_FWD_NHWC;
     kp->H = kp->Ho;
     kp->W = kp->Wo;
-    kp->C = d->kn;
-    kp->KN = d->c;
-    kp->vp = (d->kh - 1) * d->vdil - d->vpad;
-    kp->hp = (d->kw - 1) * d->hdil - d->hpad;
-    kp->Ho = d->h;
-    kp->Wo = d->w;
+    kp->C = kn;
+    kp->KN = c;
+    kp->vp = (kp->KH - 1) * kp->vd - kp->vp;
+    kp->hp = (kp->KW - 1) * kp->hd - kp->hp;
+    kp->Ho = h;
+    kp->Wo = w;
+    kp->ffd = 1;
+}
+
+int cg_dgrad_forward_form_ok(const cg_kparams *kp) {
+    return g_dgrad_fwd && kp->op == CG_DGRAD_NHWC && kp->vs == 1 && kp->hs == 1 && (kp->KH - 1) * kp->vd >= kp->vp &&
+           (kp->KW - 1) * kp->hd >= kp->hp;
 }
 
 static int weight_repack(const convgemm_desc_t *d, int elem, int *d0, int *d1, int *d2, int *perm) {
@@ -618,7 +625,7 @@ convgemm_status_t cg_prepare(const convgemm_desc_t *d, convgemm_dtype_t dtype, c
     const int repack = weight_repack(d, elem, &rd0, &rd1, &rd2, &rperm);
     /* an internal workspace is 256-byte aligned, so only caller buffers can break vec */
     kp->vec = vec_ok(elem, d, x, repack ? wsw : in);
-    if (dgrad_as_forward(d)) forward_form(d, kp);
+    if (dgrad_as_forward(d)) cg_forward_form(kp);
     long long img_a, img_o, img_b;
     image_bytes(d, ho, wo, es, &img_a, &img_o, &img_b);
     long long img_max = img_a > img_o ? img_a : img_o;
@@ -987,7 +994,7 @@ size_t convgemm_workspace_size(const convgemm_desc_t *d, convgemm_dtype_t dtype)
     kp.vs = d->vstride; kp.hs = d->hstride;
     kp.vp = d->vpad; kp.hp = d->hpad; kp.vd = d->vdil; kp.hd = d->hdil;
     kp.Ho = ho; kp.Wo = wo; kp.taps = d->kh * d->kw;
-    if (dgrad_as_forward(d)) forward_form(d, &kp);
+    if (dgrad_as_forward(d)) cg_forward_form(&kp);
     kp.P = (long long)d->b * kp.Ho * kp.Wo;
     size_t parts = 0;
     int n = cg_num_tilecfgs(elem);

@@ -14,7 +14,8 @@
  *   25    computed FLOPs (with tile padding)
  *   26    pipelined kernel;  27  stream-K;  28  waves per workgroup;  29  LDS stages
  *   30-31 raster order one-hot (order 0, the N panels of an M panel adjacent, is neither)
- *   32    the halo-tile kernel  It is trained on every timing
+ *   32    the halo-tile kernel
+ * It is trained on every timing
  * the autotuner recorded (tune.c) by the device kernels of mlp.hip --
  * forward, backward and Adam on the GPU -- and evaluated by the same forward
  * kernels to rank the candidates of a problem that was never tuned.  The
@@ -136,6 +137,7 @@ static void kp_of_key(const int *k, cg_kparams *kp, int *elem) {
     kp->Ho = convgemm_out_size(kp->H, kp->vp, kp->vd, kp->KH, kp->vs);
     kp->Wo = convgemm_out_size(kp->W, kp->hp, kp->hd, kp->KW, kp->hs);
     kp->taps = kp->KH * kp->KW;
+    if (cg_dgrad_forward_form_ok(kp)) cg_forward_form(kp); /* as the call ran it */
     kp->P = (long long)kp->B * kp->Ho * kp->Wo;
 }
 

@@ -41,10 +41,19 @@ static tentry *t_tab = NULL; /* open addressing */
 static int t_cap = 0, t_count = 0;
 static convgemm_planner_t t_mode = CONVGEMM_PLAN_TUNED;
 
+/* The key of a problem is the call's own description.  The forward form of a
+ * backward-data problem keeps the backward-data key: its timings include the
+ * read of dX (dX += ...) and must not mix with those of the forward problem
+ * that has the same GEMM. */
 void cg_key_of(int elem, const cg_kparams *kp, int *k) {
     k[0] = elem; k[1] = kp->op; k[2] = kp->vec; k[3] = kp->B; k[4] = kp->H; k[5] = kp->W; k[6] = kp->C;
     k[7] = kp->KN; k[8] = kp->KH; k[9] = kp->KW; k[10] = kp->vp; k[11] = kp->hp; k[12] = kp->vs;
     k[13] = kp->hs; k[14] = kp->vd; k[15] = kp->hd;
+    if (kp->ffd) { /* undo cg_forward_form */
+        k[1] = CG_DGRAD_NHWC; k[4] = kp->Ho; k[5] = kp->Wo; k[6] = kp->KN; k[7] = kp->C;
+        k[10] = (kp->KH - 1) * kp->vd - kp->vp;
+        k[11] = (kp->KW - 1) * kp->hd - kp->hp;
+    }
 }
 
 static uint64_t key_hash(const int *k) {

@@ -79,6 +79,7 @@ template <> struct Tiles<float> {
         {64, 128, 32, 2, 2, 3, FAM_LPW, false},   // 41
         {64, 64, 32, 2, 2, 4, FAM_LPW, false},    // 42
         {256, 128, 32, 4, 2, 2, FAM_LPW, false},  // 43 8 waves
+        {64, 64, 32, 2, 2, 2, FAM_LPW, false},    // 44 two stages: more workgroups per CU
     };
 };
 
@@ -129,6 +130,8 @@ template <> struct Tiles<uint16_t> {
         {256, 128, 64, 4, 2, 2, FAM_LPW, false},  // 42 8 waves
         {128, 256, 64, 2, 4, 2, FAM_LPW, false},  // 43 8 waves
         {128, 64, 64, 2, 2, 2, FAM_LPW, false},   // 44
+        {64, 64, 64, 2, 2, 2, FAM_LPW, false},    // 45 two stages: more workgroups per CU
+        {64, 64, 64, 2, 2, 3, FAM_LPW, false},    // 46
     };
 };
 

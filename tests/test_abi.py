@@ -283,6 +283,31 @@ def test_tuning_database_renumbers_configurations(tmp_path):
         cg.tune_clear()
 
 
+def test_backward_data_forward_form_has_its_own_key(tmp_path):
+    """Backward data of a unit-stride NHWC convolution runs on the forward kernels
+    (the forward form: dY convolved with the kernel turned by 180 degrees, c and kn
+    swapped, padding (k - 1) * dil - pad).  Its timings include reading dX (dX += ...),
+    so they are filed under the backward-data description, never under the forward
+    problem with the same GEMM: a database holding both serves each its own plan."""
+    fwd = cg.make_desc(cg.FWD, cg.NHWC, 2, 8, 8, 24, 16, 3, 3, 1, 1, 1, 1)   # c 24 -> kn 16
+    dgr = cg.make_desc(cg.DGRAD, cg.NHWC, 2, 8, 8, 16, 24, 3, 3, 1, 1, 1, 1)  # its forward form: c 24, kn 16
+    fkey = "0 0 1 2 8 8 24 16 3 3 1 1 1 1 1 1"
+    dkey = "0 4 1 2 8 8 16 24 3 3 1 1 1 1 1 1"
+    ns = cg.num_tile_shapes(cg.F32)
+    path = tmp_path / "both.tunedb"
+    path.write_text(f"# tile shapes (f32 bf16 fp8): {ns} {cg.num_tile_shapes(cg.BF16)} "
+                    f"{cg.num_tile_shapes(cg.FP8_E4M3)}\n"
+                    f"{fkey} 1 3 1 10.000\n{dkey} 1 6 2 20.000\n")
+    cg.tune_clear()
+    try:
+        cg.tune_load(str(path))
+        assert [(c, s) for c, s, _ in cg.tune_get(fwd, cg.F32)] == [(3, 1)]
+        assert [(c, s) for c, s, _ in cg.tune_get(dgr, cg.F32)] == [(6, 2)]
+        assert cg.plan(fwd)[0] == 3 and cg.plan(dgr) == (6, 2)
+    finally:
+        cg.tune_clear()
+
+
 def _taps_brute(r, pad, k, dil, s):
     # input rows i = r + s*t receive kernel tap kx when i + pad - kx*dil is a multiple of s
     return sum(1 for kx in range(k) if (r + pad - kx * dil) % s == 0)
