@@ -268,7 +268,9 @@ def batch_sweep(torch, cg, stream, n=256, reps=10, seed=5):
     """Many uneven BF16 forward layers (NHWC; batch {1,4,16}, H=W {7..56},
     Cin/Cout 32..1024, k {1,3,5}, s {1,2}): one conv_ex call per problem vs
     one batch (grouped launches captured in a hipGraph; bounded: at most 16
-    grouped GEMM launches, 18 kernels per run), plans from the
+    grouped GEMM launches, 18 kernels per run; single: every problem in ONE
+    persistent launch, with the weight repack in the run or, with static
+    weights, done once at creation), plans from the
     learned cost model shipped in convgemm_amd/data (trained on autotuner
     timings).  Returns aggregate TFLOP/s of each way of launching, and the
     learned and analytic plans' regret against exhaustive autotuning of the
@@ -311,7 +313,11 @@ def batch_sweep(torch, cg, stream, n=256, reps=10, seed=5):
         res["sequential_calls"] = {"ms": round(ms, 4), "tflops": round(flops / (ms * 1e-3) / 1e12, 2)}
         for key, fl in (("batch_graph", cg.BATCH_GRAPH),
                         ("batch_graph_persistent", cg.BATCH_GRAPH | cg.BATCH_PERSISTENT),
-                        ("batch_graph_bounded", cg.BATCH_GRAPH | cg.BATCH_BOUNDED)):
+                        ("batch_graph_bounded", cg.BATCH_GRAPH | cg.BATCH_BOUNDED),
+                        ("batch_graph_bounded_static_weights",
+                         cg.BATCH_GRAPH | cg.BATCH_BOUNDED | cg.BATCH_STATIC_WEIGHTS),
+                        ("batch_single", cg.BATCH_GRAPH | cg.BATCH_SINGLE),
+                        ("batch_single_static_weights", cg.BATCH_GRAPH | cg.BATCH_SINGLE | cg.BATCH_STATIC_WEIGHTS)):
             b = cg.Batch([cg.problem(d, w, x, y) for d, w, x, y in ops], cg.BF16, fl)
             ms = timed(lambda: b.run(stream))
             gr, kern, _ = b.info()

@@ -51,7 +51,8 @@ EXPORTED_SYMBOLS = (
     "convgemm_costmodel_save",
     "convgemm_costmodel_load", "convgemm_costmodel_predict",
     # convgemm_batch.h
-    "convgemm_batch_create", "convgemm_batch_run", "convgemm_batch_info", "convgemm_batch_destroy",
+    "convgemm_batch_create", "convgemm_batch_run", "convgemm_batch_info", "convgemm_batch_plan",
+    "convgemm_batch_destroy",
     # convgemm_explicit.h
     "im2row_nhwc", "row2im_nhwc", "im2col_nchw", "col2im_nchw", "transpose_nchw", "convgemm_gemm",
     "convgemm_im2row_ex", "convgemm_transpose_nchw_ex",
@@ -87,7 +88,7 @@ class Problem(ctypes.Structure):
                 ("scales", ctypes.c_float * 2)]
 
 
-BATCH_GRAPH, BATCH_PERSISTENT, BATCH_STATIC_WEIGHTS, BATCH_BOUNDED = 1, 2, 4, 8
+BATCH_GRAPH, BATCH_PERSISTENT, BATCH_STATIC_WEIGHTS, BATCH_BOUNDED, BATCH_SINGLE = 1, 2, 4, 8, 16
 
 _lib = None
 
@@ -184,6 +185,8 @@ def lib() -> ctypes.CDLL:
     L.convgemm_batch_run.restype = ci
     L.convgemm_batch_info.argtypes = [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci)]
     L.convgemm_batch_info.restype = ci
+    L.convgemm_batch_plan.argtypes = [vp, ci, ctypes.POINTER(ci), ctypes.POINTER(ci)]
+    L.convgemm_batch_plan.restype = ci
     L.convgemm_batch_destroy.argtypes = [vp]
     L.convgemm_batch_destroy.restype = None
     L.im2row_nhwc.argtypes = [vp, ci, vp] + [ci] * 14
@@ -573,8 +576,8 @@ def problem(desc: Desc, in_, x, out, alpha=1.0, beta=0.0, bias=None, mean=None, 
 class Batch:
     """convgemm_batch_t: many independent convolutions in grouped launches,
     optionally replayed from a hipGraph (flags BATCH_GRAPH, BATCH_PERSISTENT,
-    BATCH_STATIC_WEIGHTS, BATCH_BOUNDED).  Keep the operand tensors alive while the batch
-    is used: their addresses are bound at creation."""
+    BATCH_STATIC_WEIGHTS, BATCH_BOUNDED, BATCH_SINGLE).  Keep the operand tensors alive while
+    the batch is used: their addresses are bound at creation."""
 
     def __init__(self, problems, dtype=F32, flags=0, keep=None):
         L = lib()
@@ -599,6 +602,14 @@ class Batch:
         if st != 0:
             raise ConvGemmError(st, lib().convgemm_last_error_message().decode())
         return a.value, b.value, bool(c.value)
+
+    def plan(self, i):
+        """(cfg, splits) of problem i as given at creation; cfg -1: no implicit GEMM."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        st = lib().convgemm_batch_plan(self._h, int(i), ctypes.byref(a), ctypes.byref(b))
+        if st != 0:
+            raise ConvGemmError(st, lib().convgemm_last_error_message().decode())
+        return a.value, b.value
 
     def close(self) -> None:
         if self._h:

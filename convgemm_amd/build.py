@@ -5,7 +5,8 @@
 
 Kernels (csrc/kernels/*.hip) are compiled by hipcc for gfx950 only; the
 implicit-GEMM instantiations (csrc/kernels/slice.hip) are compiled once per
-(operand type, operation, kernel family) slice so they build in parallel.
+(operand type, operation, kernel family) slice so they build in parallel, and
+the single-launch batch kernel (csrc/kernels/single.hip) once per operand type.
 The host layer (csrc/host/*.c) is plain C compiled by gcc against the HIP
 runtime API; hipcc links both into convgemm_amd/libconvgemm.so.  Objects go
 to convgemm_amd/_build/.  build_sanitized() makes an ASan/UBSan build of the
@@ -88,7 +89,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     os.makedirs(BUILD, exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 4)
     hip_srcs = sorted(s for s in glob.glob(os.path.join(CSRC, "kernels", "*.hip"))
-                      if os.path.basename(s) != "slice.hip")
+                      if os.path.basename(s) not in ("slice.hip", "single.hip"))
     c_srcs = sorted(glob.glob(os.path.join(CSRC, "host", "*.c")))
     cmds = []
     objs = []
@@ -97,6 +98,11 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
         o = os.path.join(BUILD, f"slice_{tn}_{op}_{fam}.o")
         cmds.append([hipcc, *HIPCC_FLAGS, f"-DCG_SLICE_T={ct}", f"-DCG_SLICE_TN={tn}", f"-DCG_SLICE_OP={op}",
                      f"-DCG_SLICE_FAM={fam}", "-c", slice_src, "-o", o])
+        objs.append(o)
+    single_src = os.path.join(CSRC, "kernels", "single.hip")
+    for tn, ct in SLICE_TYPES:  # the single-launch batch kernel, one per operand type
+        o = os.path.join(BUILD, f"single_{tn}.o")
+        cmds.append([hipcc, *HIPCC_FLAGS, f"-DCG_SINGLE_T={ct}", f"-DCG_SINGLE_TN={tn}", "-c", single_src, "-o", o])
         objs.append(o)
     for s in hip_srcs:
         o = os.path.join(BUILD, os.path.basename(s) + ".o")
@@ -125,10 +131,13 @@ def build_sanitized(out_dir: str, verbose: bool = False) -> str:
     the data folder linked beside it, where the library looks for its cost model).
     Reuses the kernel objects of the normal build (building them if absent).  Load it
     with CONVGEMM_LIB=<path> and gcc's libasan preloaded."""
-    kobjs = sorted(glob.glob(os.path.join(BUILD, "slice_*.o")) + glob.glob(os.path.join(BUILD, "*.hip.o")))
+    def kernel_objs():
+        return sorted(glob.glob(os.path.join(BUILD, "slice_*.o")) + glob.glob(os.path.join(BUILD, "single_*.o")) +
+                      glob.glob(os.path.join(BUILD, "*.hip.o")))
+    kobjs = kernel_objs()
     if not kobjs:
         build(verbose=verbose)
-        kobjs = sorted(glob.glob(os.path.join(BUILD, "slice_*.o")) + glob.glob(os.path.join(BUILD, "*.hip.o")))
+        kobjs = kernel_objs()
     cc = shutil.which("gcc") or "cc"
     os.makedirs(out_dir, exist_ok=True)
     objs = []

@@ -129,6 +129,16 @@ int cg_launch_splitk_reduce_mixed(const cg_group *g, void *stream);
 long long cg_reduce_items(const cg_kparams *kp);
 /* Resident workgroups per CU of a configuration (occupancy), or 0. */
 int cg_igemm_occupancy(int elem, int cfg, const cg_kparams *kp0);
+/* The single-launch batch kernel (kernels/single.hip): forward NHWC problems
+ * with 16-byte channel chunks, each on one of a few pipelined tile shapes,
+ * all in one persistent launch.  cg_single_variant: the kernel's shape index
+ * of configuration cfg (any raster order), or -1 if the kernel does not run
+ * that shape.  cg_launch_single: g's problems, problem j on shape var[j]
+ * (device array), g->grid workgroups (a multiple of 8) walking items longest
+ * problem first.  cg_single_occupancy: resident workgroups per CU. */
+int cg_single_variant(int elem, int cfg);
+int cg_launch_single(int elem, const cg_group *g, const int *var, void *stream);
+int cg_single_occupancy(int elem);
 /* Stream-K: resident workgroups per CU of the stream-K kernel of a
  * configuration (0 if it has none), and the fix-up of the tiles that several
  * workgroups share (kp->sk workgroups, partial tiles of BM x BN floats in

@@ -13,7 +13,10 @@
  * CG_BATCH_MAX_GROUPS instances, the problems of the instances with the
  * least work move to the configuration of the largest instance of their
  * operation that serves them until the count fits, so a run is at most
- * CG_BATCH_MAX_GROUPS + 2 kernels.  Every
+ * CG_BATCH_MAX_GROUPS + 2 kernels.  With CONVGEMM_BATCH_SINGLE the forward
+ * NHWC problems with 16-byte channel chunks are planned on the shapes of the
+ * single-launch kernel (kernels/single.hip) and run, whatever their shapes,
+ * in ONE persistent launch that walks them longest first.  Every
  * problem's item range starts at a multiple of 8 so the kernels can deal each
  * problem's tiles to the 8 XCDs in contiguous runs.  One device allocation
  * holds the problem and job tables, the repacked weights and the split-K
@@ -42,12 +45,15 @@ typedef struct {
     int repack, d0, d1, d2, perm;
     const void *wsrc;
     size_t wbytes, pbytes;
+    int single; /* runs in the single-launch group */
+    int idx;    /* position in the caller's problem list */
 } bprob;
 
 typedef struct {
-    int cfg;
+    int cfg;        /* -1: the single-launch group */
     cg_kparams kp0; /* op and vec of the group */
     cg_group g;
+    const int *var; /* single-launch group: each problem's tile shape (device) */
 } glaunch;
 
 typedef struct {
@@ -76,12 +82,21 @@ struct convgemm_batch_s {
     scale_t *sc;
     hipGraph_t graph;
     hipGraphExec_t exec;
+    int n;       /* problems given at creation */
+    int *plans;  /* per given problem: cfg, splits (-1, 0: no GEMM) */
 };
 
 static convgemm_status_t errf(convgemm_status_t st, const char *msg) { return cg_set_error(st, msg); }
 
+/* The single-launch problems first, longest first; the others by kernel
+ * instance, longest first within one. */
 static int cmp_prob(const void *a, const void *b) {
     const bprob *x = (const bprob *)a, *y = (const bprob *)b;
+    if (x->single != y->single) return y->single - x->single;
+    if (x->single) {
+        if (x->cost != y->cost) return x->cost > y->cost ? -1 : 1;
+        return x->idx - y->idx;
+    }
     if (x->cfg != y->cfg) return x->cfg - y->cfg;
     if (x->kp.op != y->kp.op) return x->kp.op - y->kp.op;
     if (x->kp.vec != y->kp.vec) return x->kp.vec - y->kp.vec;
@@ -89,10 +104,62 @@ static int cmp_prob(const void *a, const void *b) {
     return 0;
 }
 
+/* The plan of a single-launch problem: a forced plan when the kernel runs
+ * its tile shape; otherwise the kernel's shape and raster order (split-K 1)
+ * with the least CU time.  Inside one launch that other problems share, a
+ * plan costs the time its workgroups hold their slots: its standalone time
+ * (tuning database, else the cost model) times the slots it fills,
+ * min(workgroups, slots); without timings, its tiles times their MFMA work
+ * with the analytic planner's small-tile penalty.  Returns the
+ * configuration, or -1 if no shape serves the problem. */
+static int plan_single(int elem, const cg_kparams *kp, long long slots) {
+    int cfg, splits;
+    if (cg_forced_config(&cfg, &splits) && splits <= 1 && cg_plan_usable(elem, cfg, kp) &&
+        cg_single_variant(elem, cfg) >= 0)
+        return cfg;
+    const int tot = cg_num_tilecfgs(elem);
+    int *cfgs = (int *)malloc(sizeof(int) * (size_t)(tot > 0 ? tot : 1));
+    int *spl = (int *)malloc(sizeof(int) * (size_t)(tot > 0 ? tot : 1));
+    float *us = (float *)malloc(sizeof(float) * (size_t)(tot > 0 ? tot : 1));
+    int n = 0, r = -1;
+    if (cfgs && spl && us) {
+        for (int c = 0; c < tot; c++)
+            if (cg_single_variant(elem, c) >= 0 && cg_plan_usable(elem, c, kp)) {
+                cfgs[n] = c;
+                spl[n++] = 1;
+            }
+        const int timed = cg_plan_times(elem, kp, cfgs, spl, n, us);
+        double best = 1e300;
+        for (int i = 0; i < n; i++) {
+            cg_tilecfg t;
+            if (cg_get_tilecfg(elem, cfgs[i], &t)) continue;
+            cg_kparams k = *kp;
+            cg_fill_tiles(&k, &t, 1);
+            const long long w = (long long)k.mtiles * k.ntiles;
+            double cost;
+            if (timed) {
+                if (us[i] < 0.f) continue; /* not timed by the database */
+                cost = (double)us[i] * (double)(w < slots ? w : slots);
+            } else {
+                cost = (double)w * t.BM * t.BN * (double)k.KT * t.BK;
+                cost *= 1.0 + 48.0 / ((double)t.BM * t.BN / (t.BM + t.BN));
+            }
+            if (cost < best) {
+                best = cost;
+                r = cfgs[i];
+            }
+        }
+    }
+    free(cfgs);
+    free(spl);
+    free(us);
+    return r;
+}
+
 /* Plan one problem.  *kind: 1 = implicit GEMM (bp filled), 2 = beta scaling
  * only (sc filled, alpha == 0), 0 = no-op. */
-static convgemm_status_t plan_one(const convgemm_problem_t *q, convgemm_dtype_t dtype, bprob *bp, scale_t *sc,
-                                  int *kind) {
+static convgemm_status_t plan_one(const convgemm_problem_t *q, convgemm_dtype_t dtype, unsigned flags,
+                                  long long slots, bprob *bp, scale_t *sc, int *kind) {
     const convgemm_desc_t *d = &q->desc;
     cg_kparams kp;
     int elem, chunk;
@@ -147,9 +214,14 @@ static convgemm_status_t plan_one(const convgemm_problem_t *q, convgemm_dtype_t 
     bp->wsrc = q->in;
     bp->wbytes = bp->repack ? (size_t)(K * d->kn * es) : 0;
 
-    int cfg, splits;
-    cg_choose_plan(elem, &kp, &cfg, &splits);
-    if (splits < 1) splits = 1; /* stream-K plans run as plain tiles in a grouped launch */
+    int cfg = -1, splits = 1;
+    if ((flags & CONVGEMM_BATCH_SINGLE) && kp.op == CG_FWD_NHWC && kp.vec) cfg = plan_single(elem, &kp, slots);
+    if (cfg >= 0) {
+        bp->single = 1;
+    } else {
+        cg_choose_plan(elem, &kp, &cfg, &splits);
+        if (splits < 1) splits = 1; /* stream-K plans run as plain tiles in a grouped launch */
+    }
     cg_tilecfg t;
     if (cfg < 0 || cg_get_tilecfg(elem, cfg, &t))
         return errf(CONVGEMM_ERR_UNSUPPORTED, "no kernel configuration for a batch problem");
@@ -158,7 +230,9 @@ static convgemm_status_t plan_one(const convgemm_problem_t *q, convgemm_dtype_t 
     bp->cfg = cfg;
     bp->pbytes = cg_partial_bytes(&kp);
     bp->items = (long long)kp.mtiles * kp.ntiles * kp.splits * kp.ncls_h * kp.ncls_w;
-    bp->cost = (kp.KT + kp.splits - 1) / kp.splits;
+    /* the ordering key: K tiles per item; single-launch items, whose tile shapes
+     * differ, by their MFMA work */
+    bp->cost = bp->single ? (long long)kp.KT * t.BM * t.BN : (kp.KT + kp.splits - 1) / kp.splits;
     *kind = 1;
     return CONVGEMM_SUCCESS;
 }
@@ -168,9 +242,12 @@ static convgemm_status_t issue(convgemm_batch_t bt, hipStream_t s) {
     const ewlaunch *w = &bt->ew_run;
     if (w->njobs && (e = cg_launch_ew_grouped(w->jobs, w->start, w->njobs, w->total, w->grid, cg_esize(bt->elem), s)))
         return cg_hip_fail(e, "weight repacks and scalings");
-    for (int i = 0; i < bt->ngl; i++)
-        if ((e = cg_launch_igemm_grouped(bt->elem, bt->gl[i].cfg, &bt->gl[i].kp0, &bt->gl[i].g, s)))
+    for (int i = 0; i < bt->ngl; i++) {
+        const glaunch *g = &bt->gl[i];
+        if (g->cfg < 0 ? (e = cg_launch_single(bt->elem, &g->g, g->var, s))
+                       : (e = cg_launch_igemm_grouped(bt->elem, g->cfg, &g->kp0, &g->g, s)))
             return cg_hip_fail(e, "grouped implicit GEMM");
+    }
     if (bt->has_red && (e = cg_launch_splitk_reduce_mixed(&bt->red, s)))
         return cg_hip_fail(e, "grouped split-K reduction");
     return CONVGEMM_SUCCESS;
@@ -183,10 +260,12 @@ static void release(convgemm_batch_t bt) {
     if (bt->dmem) hipFree(bt->dmem);
     free(bt->gl);
     free(bt->sc);
+    free(bt->plans);
     free(bt);
 }
 
 static int same_instance(const bprob *x, const bprob *y) {
+    if (x->single || y->single) return x->single == y->single;
     return x->cfg == y->cfg && x->kp.op == y->kp.op && x->kp.vec == y->kp.vec;
 }
 
@@ -277,7 +356,7 @@ convgemm_status_t convgemm_batch_create(const convgemm_problem_t *problems, int 
     *batch = NULL;
     if (n < 0 || (n > 0 && !problems)) return errf(CONVGEMM_ERR_INVALID_VALUE, "invalid problem list");
     if (flags & ~(unsigned)(CONVGEMM_BATCH_GRAPH | CONVGEMM_BATCH_PERSISTENT | CONVGEMM_BATCH_STATIC_WEIGHTS |
-                            CONVGEMM_BATCH_BOUNDED))
+                            CONVGEMM_BATCH_BOUNDED | CONVGEMM_BATCH_SINGLE))
         return errf(CONVGEMM_ERR_INVALID_VALUE, "unknown batch flags");
     if (dtype != CONVGEMM_F32 && dtype != CONVGEMM_BF16 && dtype != CONVGEMM_FP8_E4M3)
         return errf(CONVGEMM_ERR_INVALID_VALUE, "unknown dtype");
@@ -295,22 +374,37 @@ convgemm_status_t convgemm_batch_create(const convgemm_problem_t *problems, int 
     bt->elem = dtype == CONVGEMM_BF16 ? CG_ELEM_BF16 : dtype == CONVGEMM_FP8_E4M3 ? CG_ELEM_FP8 : CG_ELEM_F32;
     bt->flags = flags;
     bt->sc = (scale_t *)calloc((size_t)(n > 0 ? n : 1), sizeof(scale_t));
-    if (!bt->sc) {
+    bt->plans = (int *)malloc(sizeof(int) * 2 * (size_t)(n > 0 ? n : 1));
+    bt->n = n;
+    if (!bt->sc || !bt->plans) {
         st = errf(CONVGEMM_ERR_ALLOC, "host allocation failed");
         goto fail;
     }
 
     /* ---- plan every problem ---- */
+    int dev = 0, ncu = 0;
+    hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 256;
+    const int socc = (flags & CONVGEMM_BATCH_SINGLE) ? cg_single_occupancy(bt->elem) : 0;
+    const long long slots = (long long)(socc > 0 ? socc : 1) * ncu; /* resident single-launch workgroups */
     int np = 0;
     for (int i = 0; i < n; i++) {
         int kind;
-        st = plan_one(&problems[i], dtype, &bp[np], &bt->sc[bt->nsc], &kind);
+        bt->plans[2 * i] = -1;
+        bt->plans[2 * i + 1] = 0;
+        st = plan_one(&problems[i], dtype, flags, slots, &bp[np], &bt->sc[bt->nsc], &kind);
         if (st != CONVGEMM_SUCCESS) goto fail;
-        if (kind == 1) np++;
+        if (kind == 1) bp[np++].idx = i;
         else if (kind == 2) bt->nsc++;
     }
     qsort(bp, (size_t)np, sizeof(bprob), cmp_prob);
-    if (flags & CONVGEMM_BATCH_BOUNDED) consolidate(bt->elem, bp, np);
+    int nsingle = 0;
+    while (nsingle < np && bp[nsingle].single) nsingle++;
+    if (flags & CONVGEMM_BATCH_BOUNDED) consolidate(bt->elem, bp + nsingle, np - nsingle);
+    for (int i = 0; i < np; i++) {
+        bt->plans[2 * bp[i].idx] = bp[i].cfg;
+        bt->plans[2 * bp[i].idx + 1] = bp[i].kp.splits;
+    }
 
     /* ---- groups, item ranges, device layout ---- */
     int ngl = 0, nsplit = 0, nrep = 0;
@@ -325,7 +419,7 @@ convgemm_status_t convgemm_batch_create(const convgemm_problem_t *problems, int 
     const int njobs = nrep + bt->nsc;
     const size_t tab_bytes = AL256(sizeof(cg_kparams) * (size_t)(np + nsplit));
     const size_t job_bytes = AL256(sizeof(cg_ewjob) * (size_t)(njobs > 0 ? njobs : 1));
-    const int nstart = np + ngl + nsplit + 1 + (njobs + 1) + (nrep + 1);
+    const int nstart = np + ngl + nsplit + 1 + (njobs + 1) + (nrep + 1) + nsingle;
     const size_t start_bytes = AL256(sizeof(int) * (size_t)nstart);
     size_t wtot = 0, ptot = 0;
     for (int i = 0; i < np; i++) {
@@ -387,15 +481,12 @@ convgemm_status_t convgemm_batch_create(const convgemm_problem_t *problems, int 
         }
     }
     /* implicit-GEMM groups: item ranges padded to multiples of 8 */
-    int dev = 0, ncu = 0;
-    hipGetDevice(&dev);
-    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 256;
     int so = 0; /* offset into hstart */
     for (int i = 0; i < np;) {
         int j = i;
         while (j < np && same_instance(&bp[j], &bp[i])) j++;
         glaunch *g = &bt->gl[bt->ngl++];
-        g->cfg = bp[i].cfg;
+        g->cfg = bp[i].single ? -1 : bp[i].cfg;
         g->kp0 = bp[i].kp;
         long long acc = 0;
         for (int k = i; k < j; k++) {
@@ -412,7 +503,11 @@ convgemm_status_t convgemm_batch_create(const convgemm_problem_t *problems, int 
         g->g.nprob = j - i;
         g->g.total = (int)acc;
         g->g.grid = (int)acc;
-        if (flags & CONVGEMM_BATCH_PERSISTENT) {
+        if (g->cfg < 0) { /* the single launch: persistent, a multiple of 8 workgroups */
+            long long cap = slots & ~7LL;
+            if (cap < 8) cap = 8;
+            if (cap < g->g.grid) g->g.grid = (int)cap;
+        } else if (flags & CONVGEMM_BATCH_PERSISTENT) {
             int occ = cg_igemm_occupancy(bt->elem, g->cfg, &g->kp0);
             long long cap = (long long)(occ > 0 ? occ : 1) * ncu;
             cap &= ~7LL;
@@ -420,6 +515,11 @@ convgemm_status_t convgemm_batch_create(const convgemm_problem_t *problems, int 
             if (cap < g->g.grid) g->g.grid = (int)cap;
         }
         so += j - i + 1;
+        if (g->cfg < 0) { /* each problem's tile shape */
+            for (int k = i; k < j; k++) hstart[so + (k - i)] = cg_single_variant(bt->elem, bp[k].cfg);
+            g->var = dstart + so;
+            so += j - i;
+        }
         i = j;
     }
     /* split-K reductions of every operation: one grouped launch */
@@ -559,6 +659,14 @@ convgemm_status_t convgemm_batch_info(convgemm_batch_t bt, int *groups, int *ker
     if (groups) *groups = bt->ngl;
     if (kernels_per_run) *kernels_per_run = (bt->ew_run.njobs > 0) + bt->ngl + bt->has_red;
     if (graph) *graph = bt->exec != NULL;
+    return cg_set_error(CONVGEMM_SUCCESS, "");
+}
+
+convgemm_status_t convgemm_batch_plan(convgemm_batch_t bt, int i, int *cfg, int *splits) {
+    if (!bt) return errf(CONVGEMM_ERR_INVALID_VALUE, "batch is NULL");
+    if (i < 0 || i >= bt->n) return errf(CONVGEMM_ERR_INVALID_VALUE, "problem index out of range");
+    if (cfg) *cfg = bt->plans[2 * i];
+    if (splits) *splits = bt->plans[2 * i + 1];
     return cg_set_error(CONVGEMM_SUCCESS, "");
 }
 

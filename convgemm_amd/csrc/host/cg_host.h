@@ -26,6 +26,11 @@ int cg_halo_fits(const cg_kparams *kp, const cg_tilecfg *t);
 int cg_plan_usable(int elem, int cfg, const cg_kparams *kp);
 /* The analytic ("no learning") planner. */
 void cg_default_plan(int elem, const cg_kparams *base, int *cfg, int *splits);
+/* Times of given candidates from the planner's evidence (tuning database
+ * timings, else the cost model; 1 if us[] was filled). */
+int cg_plan_times(int elem, const cg_kparams *kp, const int *cfgs, const int *splits, int n, float *us);
+/* The plan forced through cg_force_config (1), or none (0). */
+int cg_forced_config(int *cfg, int *splits);
 size_t cg_partial_bytes(const cg_kparams *kp);
 /* Plan chosen by the last call on this thread: tile configuration, split-K
  * factor and number of kernels launched. */

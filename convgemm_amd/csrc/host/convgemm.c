@@ -108,6 +108,12 @@ void cg_force_config(int cfg, int splits) {
     g_force_splits = splits;
 }
 
+int cg_forced_config(int *cfg, int *splits) {
+    *cfg = g_force_cfg;
+    *splits = g_force_splits;
+    return g_force_cfg >= 0;
+}
+
 void cg_set_plan_hook(cg_plan_hook_t hook, void *ctx) {
     g_plan_hook = hook;
     g_plan_hook_ctx = ctx;

@@ -416,6 +416,16 @@ static int predict(int elem, const cg_kparams *kp, const int *cfgs, const int *s
     return 0;
 }
 
+/* Predicted times of n candidates of one problem; 0, or -1 without a model
+ * (runs the model's device kernels: not for a capturing stream). */
+int cg_costmodel_times(int elem, const cg_kparams *kp, const int *cfgs, const int *splits, int n, float *us) {
+    pthread_mutex_lock(&m_lock);
+    const int ready = g_model.ready;
+    pthread_mutex_unlock(&m_lock);
+    if (!ready || n <= 0) return -1;
+    return predict(elem, kp, cfgs, splits, n, us) ? -1 : 0;
+}
+
 int cg_costmodel_choose(int elem, const cg_kparams *kp, int *cfg, int *splits, int allow_device) {
     int key[16];
     cg_key_of(elem, kp, key);

@@ -228,6 +228,39 @@ static void plan_hook(void *ctx, int elem, const cg_kparams *kp, int *cfg, int *
         cg_costmodel_choose(elem, kp, cfg, splits, !cg_planning_in_capture());
 }
 
+int cg_costmodel_times(int elem, const cg_kparams *kp, const int *cfgs, const int *splits, int n, float *us);
+
+/* Times of n candidates (cfgs[i], splits[i]) of one problem, from the
+ * evidence the planner would use: the tuning database's recorded timings when
+ * it holds the problem (a candidate it did not time gets a negative time),
+ * else the cost model's predictions.  Returns 1 with us[] filled, 0 when
+ * neither is available (or the stream is being captured and the model would
+ * have to run). */
+int cg_plan_times(int elem, const cg_kparams *kp, const int *cfgs, const int *splits, int n, float *us) {
+    if (n <= 0) return 0;
+    if (t_mode != CONVGEMM_PLAN_ANALYTIC && t_mode != CONVGEMM_PLAN_MODEL) {
+        int k[KEYLEN], hit = 0;
+        cg_key_of(elem, kp, k);
+        pthread_mutex_lock(&t_lock);
+        if (t_cap) {
+            const tentry *e = find_slot(t_tab, t_cap, k);
+            if (e->used) {
+                for (int i = 0; i < n; i++) {
+                    us[i] = -1.f;
+                    for (int j = 0; j < e->n; j++)
+                        if (e->cfg[j] == cfgs[i] && e->splits[j] == splits[i]) us[i] = e->us[j];
+                    hit |= us[i] >= 0.f;
+                }
+            }
+        }
+        pthread_mutex_unlock(&t_lock);
+        if (hit) return 1;
+    }
+    if ((t_mode == CONVGEMM_PLAN_LEARNED || t_mode == CONVGEMM_PLAN_MODEL) && !cg_planning_in_capture())
+        return cg_costmodel_times(elem, kp, cfgs, splits, n, us) == 0;
+    return 0;
+}
+
 void convgemm_set_planner(convgemm_planner_t mode) {
     t_mode = mode;
     cg_set_plan_hook(plan_hook, NULL);

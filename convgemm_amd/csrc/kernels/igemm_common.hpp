@@ -175,6 +175,10 @@ __device__ inline long long out_addr(const cg_kparams &p, const Ctx &x, int m, i
 // ReLU.  `ev` holds the channel's (bias, mean, inv_std, gamma, bn_beta).
 template <int OP>
 __device__ inline float epi_value(const cg_kparams &p, float acc, float old, const float (&ev)[5]) {
+    // Every product and sum rounded on its own: whether the compiler fuses a
+    // multiply and an add depends on the code around the inlined call, and
+    // every kernel that writes a given output must round it the same way.
+#pragma clang fp contract(off)
     float t = p.alpha * p.scale * acc;
     if (p.beta != 0.0f) t += p.beta * old;
     if constexpr (OP == CG_FWD_NHWC || OP == CG_FWD_NCHW) {

@@ -38,9 +38,37 @@ template <int N> __device__ __attribute__((always_inline)) inline void wait_vm()
 
 typedef __attribute__((address_space(3))) void lds_void_t;
 
-template <typename T, int BM, int BN, int WM, int WN, int NS>
+// Order of the K steps of a forward tile, a step being (tap, block of
+// channels): the KH x KW taps of one channel block run back to back, so the
+// input rows one tap gathers are gathered again by the next taps while they
+// are still in L2 (with the taps of one block outermost instead, the NCHW
+// 3x3 stride-2 layer at batch 32 fetched 4.4x its compulsory bytes).  A
+// split-K slice or a stream-K range is a contiguous run of steps.
+__device__ __attribute__((always_inline)) inline void step_start(const cg_kparams &p, int s, int &tap, int &cb) {
+    cb = s / p.taps;
+    tap = s - cb * p.taps;
+}
+// Advance (tap, cb) and the tap's (kx, ky) = (tap / KW, tap mod KW) by one step.
+__device__ __attribute__((always_inline)) inline void step_next(const cg_kparams &p, int &tap, int &cb, int &kx,
+                                                                int &ky) {
+    ++tap;
+    if (++ky == p.KW) {
+        ky = 0;
+        if (++kx == p.KH) {
+            kx = 0;
+            tap = 0;
+            ++cb;
+        }
+    }
+}
+
+// EXT: the NS stages live at ext + s * STAGE (a kernel that runs several tile
+// shapes gives them one LDS buffer of the largest size); otherwise in LDS
+// objects of the tile's own.
+template <typename T, int BM, int BN, int WM, int WN, int NS, bool EXT = false>
 __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &p, const int lin, const int sk_k0 = -1,
-                                                         const int sk_k1 = 0, float *piece = nullptr) {
+                                                         const int sk_k1 = 0, float *piece = nullptr,
+                                                         char *ext = nullptr) {
     static_assert(sizeof(T) == 4 || sizeof(T) == 2 || sizeof(T) == 1, "FP32, BF16 or FP8 operands");
     constexpr int ES = sizeof(T);
     constexpr int RB = 128;                 // bytes per row per stage
@@ -117,14 +145,15 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
     const int nsteps = kt_end - kt_begin;
 
     // issue iterator: the next K step to load (tap, channel block)
-    int i_tap = kt_begin / p.kchunks;
-    int i_cb = kt_begin - i_tap * p.kchunks;
+    int i_tap, i_cb;
+    step_start(p, kt_begin, i_tap, i_cb);
     int i_kx = i_tap / p.KW;
     int i_ky = i_tap - i_kx * p.KW;
 
     auto stage_ptr = [&](auto s) __attribute__((always_inline)) -> char * {
         constexpr int S = decltype(s)::value;
-        if constexpr (S == 0) return st0;
+        if constexpr (EXT) return ext + S * STAGE;
+        else if constexpr (S == 0) return st0;
         else if constexpr (S == 1) return st1;
         else if constexpr (S == 2) return st2;
         else return st3;
@@ -166,13 +195,7 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
             }
         }
     };
-    auto advance = [&]() __attribute__((always_inline)) {
-        if (++i_cb == p.kchunks) {
-            i_cb = 0;
-            ++i_tap;
-            if (++i_ky == p.KW) { i_ky = 0; ++i_kx; }
-        }
-    };
+    auto advance = [&]() __attribute__((always_inline)) { step_next(p, i_tap, i_cb, i_kx, i_ky); };
     auto issue = [&](auto s, bool live) __attribute__((always_inline)) {
         issue_part(s, live, std::integral_constant<int, 0>{}, std::integral_constant<int, LPS>{});
         advance();
@@ -383,7 +406,7 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
     if (epi_staged<CG_FWD_NHWC>(p)) {
         static_assert(5 * BN * sizeof(float) <= STAGE, "epilogue vectors fit one LDS stage");
         __syncthreads();
-        s_epi = reinterpret_cast<float *>(st0);
+        s_epi = reinterpret_cast<float *>(stage_ptr(std::integral_constant<int, 0>{}));
         epi_stage(p, nt * BN, BN, s_epi, tid, NW * 64);
         __syncthreads();
     }

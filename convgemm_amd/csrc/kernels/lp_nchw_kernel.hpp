@@ -95,8 +95,8 @@ __device__ __attribute__((always_inline)) inline void lpn_tile(const cg_kparams 
     const int kt_end = skm ? sk_k1 : min(KT, kt_begin + per_split);
     const int nsteps = kt_end - kt_begin;
 
-    int i_tap = kt_begin / p.kchunks;
-    int i_cb = kt_begin - i_tap * p.kchunks;
+    int i_tap, i_cb;
+    step_start(p, kt_begin, i_tap, i_cb);
     int i_kx = i_tap / p.KW;
     int i_ky = i_tap - i_kx * p.KW;
 
@@ -144,13 +144,7 @@ __device__ __attribute__((always_inline)) inline void lpn_tile(const cg_kparams 
             }
         }
     };
-    auto advance = [&]() __attribute__((always_inline)) {
-        if (++i_cb == p.kchunks) {
-            i_cb = 0;
-            ++i_tap;
-            if (++i_ky == p.KW) { i_ky = 0; ++i_kx; }
-        }
-    };
+    auto advance = [&]() __attribute__((always_inline)) { step_next(p, i_tap, i_cb, i_kx, i_ky); };
     auto issue = [&](auto s, bool live) __attribute__((always_inline)) {
         issue_part(s, live, std::integral_constant<int, 0>{}, std::integral_constant<int, LPS>{});
         advance();

@@ -177,6 +177,23 @@ template <> struct Tiles<uint8_t> {
 
 template <typename T> constexpr int num_tiles() { return (int)(sizeof(Tiles<T>::defs) / sizeof(TileDef)); }
 
+// Tile shapes of the single-launch batch kernel (single.hip): 4-wave
+// pipelined tiles whose NS stages fit SINGLE_LDS, so that two workgroups
+// share a CU.  Entries are tile-shape ids of Tiles<T>; the kernel switches
+// over them per problem.
+constexpr int SINGLE_LDS = 80 * 1024;
+template <typename T> struct SingleSet;
+template <> struct SingleSet<float> {
+    static constexpr int ids[] = {19, 21, 13, 20};  // 64x64, 128x64, 128x128, 256x64
+};
+template <> struct SingleSet<uint16_t> {
+    static constexpr int ids[] = {13, 15, 16, 17, 18};  // 64x64, 128x64, 64x128, 128x128, 256x64
+};
+template <> struct SingleSet<uint8_t> {
+    static constexpr int ids[] = {12, 14, 15, 16, 17};  // 64x64, 128x64, 64x128, 128x128, 256x64
+};
+template <typename T> constexpr int num_single() { return (int)(sizeof(SingleSet<T>::ids) / sizeof(int)); }
+
 // Signature of one instantiation slice: the kernels of one (operand type,
 // operation, kernel family), compiled as a translation unit of its own.
 //   g == nullptr: one problem (kp); g != nullptr: the grouped kernel over g's

@@ -23,6 +23,10 @@ using namespace cg;
 CG_SLICE_DECL_T(f32)
 CG_SLICE_DECL_T(bf16)
 CG_SLICE_DECL_T(fp8)
+// the single-launch batch kernels (single.hip), one per operand type
+extern "C" __attribute__((weak)) int cg_single_f32(const cg_group *, const int *, int *, void *);
+extern "C" __attribute__((weak)) int cg_single_bf16(const cg_group *, const int *, int *, void *);
+extern "C" __attribute__((weak)) int cg_single_fp8(const cg_group *, const int *, int *, void *);
 
 namespace {
 
@@ -78,7 +82,46 @@ int launch_any(int elem, int cfg, const cg_kparams *kp, const cg_group *g, int *
     return f(base, kp, g, occ, stream);
 }
 
+template <typename T> int single_index(int base) {
+    for (int v = 0; v < num_single<T>(); v++)
+        if (SingleSet<T>::ids[v] == base) return v;
+    return -1;
+}
+
+typedef int (*single_fn)(const cg_group *, const int *, int *, void *);
+single_fn single_of(int elem) {
+    switch (elem) {
+    case CG_ELEM_F32: return cg_single_f32;
+    case CG_ELEM_BF16: return cg_single_bf16;
+    case CG_ELEM_FP8: return cg_single_fp8;
+    }
+    return nullptr;
+}
+
 }  // namespace
+
+extern "C" int cg_single_variant(int elem, int cfg) {
+    int base;
+    if (!def_of(elem, cfg, &base)) return -1;
+    switch (elem) {
+    case CG_ELEM_F32: return single_index<float>(base);
+    case CG_ELEM_BF16: return single_index<uint16_t>(base);
+    case CG_ELEM_FP8: return single_index<uint8_t>(base);
+    }
+    return -1;
+}
+
+extern "C" int cg_launch_single(int elem, const cg_group *g, const int *var, void *stream) {
+    const single_fn f = single_of(elem);
+    return f ? f(g, var, nullptr, stream) : (int)hipErrorInvalidValue;
+}
+
+extern "C" int cg_single_occupancy(int elem) {
+    const single_fn f = single_of(elem);
+    int occ = 0;
+    if (!f || f(nullptr, nullptr, &occ, nullptr) != 0) return 0;
+    return occ;
+}
 
 extern "C" int cg_num_tilecfgs(int elem) { return num_shapes(elem) * CG_NUM_RASTERS; }
 

@@ -10,7 +10,9 @@
  * CONVGEMM_BATCH_GRAPH the launches are captured once into a hipGraph at
  * creation and every run replays it.  A run is one element-wise launch
  * (weight repacks, beta-only scalings), the grouped implicit GEMMs and one
- * split-K reduction.
+ * split-K reduction.  With CONVGEMM_BATCH_SINGLE the forward NHWC problems
+ * run in ONE persistent launch whatever their tile shapes; with static
+ * weights and a graph, a batch of such problems is one kernel per run.
  */
 #ifndef CONVGEMM_AMD_CONVGEMM_BATCH_H
 #define CONVGEMM_AMD_CONVGEMM_BATCH_H
@@ -52,7 +54,16 @@ enum {
      * operation.  Those problems' outputs then differ from a per-call run in
      * summation order (within the operand type's accumulation tolerance);
      * without this flag every output is bit-identical to a per-call run. */
-    CONVGEMM_BATCH_BOUNDED = 8
+    CONVGEMM_BATCH_BOUNDED = 8,
+    /* one persistent launch for every forward NHWC problem whose channels
+     * come in 16-byte chunks (FP32: C % 4 == 0, BF16: C % 8, FP8: C % 16):
+     * each is planned on one of the tile shapes of the single-launch kernel
+     * (4-wave pipelined tiles of 64x64 to 256x64, any raster order, no
+     * split-K) and the kernel's workgroups walk the tiles of all of them,
+     * longest K first.  Other problems are grouped as without the flag.
+     * Outputs are bit-identical to per-call runs of the same plans
+     * (convgemm_batch_plan). */
+    CONVGEMM_BATCH_SINGLE = 16
 };
 
 /* Plan the n problems, allocate the batch's device workspace (repacked
@@ -73,6 +84,11 @@ convgemm_status_t convgemm_batch_run(convgemm_batch_t batch, convgemm_stream_t s
 /* Launch structure: grouped implicit-GEMM launches, kernels per run (all
  * kinds), and whether runs replay a graph.  Any pointer may be NULL. */
 convgemm_status_t convgemm_batch_info(convgemm_batch_t batch, int *groups, int *kernels_per_run, int *graph);
+
+/* The plan problem i (its index in the list given at creation) runs: tile
+ * configuration and split-K factor as convgemm_plan reports them; cfg -1 for
+ * a problem with no implicit GEMM (a no-op, or alpha == 0). */
+convgemm_status_t convgemm_batch_plan(convgemm_batch_t batch, int i, int *cfg, int *splits);
 
 /* Waits for the device, then frees the batch. NULL is ignored. */
 void convgemm_batch_destroy(convgemm_batch_t batch);

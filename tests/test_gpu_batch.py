@@ -332,3 +332,104 @@ def test_bounded_batch(dtype, tmp_path):
             same += bool(np.array_equal(got, exp))
         assert same >= len(cases) // 2, same
     b.close()
+
+
+# ---- CONVGEMM_BATCH_SINGLE: forward NHWC problems in one persistent launch ----
+
+CHUNK = {cg.F32: 4, cg.BF16: 8, cg.FP8_E4M3: 16}  # channels per 16-byte chunk
+
+
+def single_cases(dtype, n, seed):
+    """Forward NHWC problems with 16-byte channel chunks: ragged M/N/K tails,
+    stride 2, 1x1 to 5x5, epilogues, alpha/beta."""
+    rng = np.random.default_rng(seed)
+    q = CHUNK[dtype]
+    out = []
+    for i in range(n):
+        k = int(rng.choice([1, 3, 5]))
+        s = int(rng.choice([1, 2]))
+        h = int(rng.choice([7, 12, 20, 28]))
+        c = q * int(rng.integers(1, 160 // q + 1))
+        kn = int(rng.choice(range(16, 193, 8)))
+        out.append(Case(D(cg.FWD, cg.NHWC, int(rng.choice([1, 2, 4])), h, h, c, kn, k, s=s), dtype, seed + i,
+                        alpha=0.75 if i % 5 == 1 else 1.0, beta=1.0 if i % 5 == 3 else 0.0,
+                        epi=BN_RELU if i % 2 == 0 else None))
+    return out
+
+
+def forced_reference(c, cfg, splits):
+    """The per-call result of plan (cfg, splits)."""
+    ref = c.out0.clone()
+    cg.force_config(cfg, splits)
+    try:
+        cg.conv_ex(c.desc, c.in_, c.x, ref, alpha=c.alpha, beta=c.beta, dtype=c.dtype, scales=c.scales, **c.epi)
+        assert cg.last_plan()[:2] == (cfg, splits)
+    finally:
+        cg.force_config(-1, 0)
+    return ref
+
+
+def run_and_check(b, cases, refs, tag, reps):
+    for rep in range(reps):
+        for c in cases:
+            c.reset()  # eager fills between replays
+        torch.cuda.synchronize()
+        b.run()
+        torch.cuda.synchronize()
+        check(cases, refs, f"{tag} replay={rep}")
+
+
+@pytest.mark.parametrize("dtype", [cg.F32, cg.BF16, cg.FP8_E4M3])
+@pytest.mark.parametrize("flags", [cg.BATCH_SINGLE, cg.BATCH_SINGLE | cg.BATCH_GRAPH | cg.BATCH_STATIC_WEIGHTS])
+def test_single_launch_batch(dtype, flags):
+    """Every eligible problem on a tile shape of the single-launch kernel, split-K 1;
+    with static weights and a graph the run is ONE kernel; outputs bit-identical to
+    per-call runs of the batch's plans."""
+    L = cg.lib()
+    cases = single_cases(dtype, 40, 5000 + 100 * int(dtype))
+    b = cg.Batch([c.problem() for c in cases], dtype, flags)
+    groups, kernels, graph = b.info()
+    plans = [b.plan(i) for i in range(len(cases))]
+    for (cfg, splits), c in zip(plans, cases):
+        assert L.cg_single_variant(int(dtype), cfg) >= 0 and splits == 1, (cfg, splits, c.desc)
+    assert groups == 1
+    if flags & cg.BATCH_STATIC_WEIGHTS:
+        assert kernels == 1 and graph, (groups, kernels, graph)
+    refs = [forced_reference(c, cfg, splits) for c, (cfg, splits) in zip(cases, plans)]
+    torch.cuda.synchronize()
+    run_and_check(b, cases, refs, f"single dtype={dtype} flags={flags}", 3 if flags & cg.BATCH_GRAPH else 1)
+    b.close()
+
+
+@pytest.mark.parametrize("dtype", [cg.F32, cg.BF16, cg.FP8_E4M3])
+def test_single_launch_every_shape(dtype):
+    """Each tile shape of the single-launch kernel forced in turn (raster orders 0
+    and 2), beside problems the kernel does not serve (NCHW, 8-byte channel rows,
+    alpha = 0), which run in the usual grouped launches."""
+    L = cg.lib()
+    ns = cg.num_tile_shapes(dtype)
+    shapes = [k for k in range(ns) if L.cg_single_variant(int(dtype), k) >= 0]
+    assert len(shapes) >= 4
+    q = CHUNK[dtype]
+    for base in shapes:
+        for raster in (0, 2):
+            cfg = base + raster * ns
+            print(f"single shape dtype {dtype} cfg {cfg}", flush=True)
+            cases = single_cases(dtype, 6, 7000 + cfg)
+            others = [Case(D(cg.FWD, cg.NCHW, 2, 9, 9, 16, 24, 3), dtype, 31, epi=BN_RELU),
+                      Case(D(cg.FWD, cg.NHWC, 2, 8, 8, 16, 16, 3), dtype, 32, alpha=0.0, beta=2.0)]
+            if dtype != cg.FP8_E4M3:
+                others.append(Case(D(cg.FWD, cg.NHWC, 1, 10, 10, q + q // 2, 40, 3), dtype, 33))
+            allc = cases + others
+            cg.force_config(cfg, 1)
+            try:
+                b = cg.Batch([c.problem() for c in allc], dtype, cg.BATCH_SINGLE | cg.BATCH_GRAPH)
+            finally:
+                cg.force_config(-1, 0)
+            plans = [b.plan(i) for i in range(len(allc))]
+            assert all(p == (cfg, 1) for p in plans[:len(cases)]), plans
+            assert plans[len(cases) + 1][0] == -1  # alpha = 0: a scaling only
+            refs = [forced_reference(c, *p) if p[0] >= 0 else c.reference() for c, p in zip(allc, plans)]
+            torch.cuda.synchronize()
+            run_and_check(b, allc, refs, f"single cfg={cfg}", 2)
+            b.close()

@@ -173,16 +173,47 @@ def test_uneven_batch_256_learned_plans():
         fl = [p["b"] * p["c"] * p["kn"] * p["k"] ** 2 * ((p["h"] + 2 * p["pad"] - p["k"]) // p["s"] + 1) ** 2
               for p in probs]
         order = [int(np.argmax(fl))] + [i for i in range(0, 256, 17) if i != int(np.argmax(fl))][:15]
-        rng = np.random.default_rng(51)
-        for i in order:
-            c, p = cases[i], cases[i]["p"]
-            s = Shape(p["b"], p["h"], p["h"], p["c"], p["kn"], p["k"], p["k"], p["pad"], p["pad"], p["s"], p["s"],
-                      1, 1)
-            ys = np.stack([rng.integers(0, n, 64) for n in (s.b, s.ho, s.wo, s.kn)], 1)
-            ref = sampled.fwd_sampled(NHWC, s, c["w"].float().cpu().numpy(), c["x"].float().cpu().numpy(), ys)
-            got = sampled.nhwc_index(NHWC, c["y"].cpu().numpy(), ys)
-            e = _err(got, ref)
-            print(f"  problem {i} {p}: vs FP64 on the BF16 operands {e:.2e}", flush=True)
-            assert e <= TOL_DEQ[cg.BF16], (i, p, e)
+
+        def oracle_check(tag):
+            rng = np.random.default_rng(51)
+            for i in order:
+                c, p = cases[i], cases[i]["p"]
+                s = Shape(p["b"], p["h"], p["h"], p["c"], p["kn"], p["k"], p["k"], p["pad"], p["pad"], p["s"],
+                          p["s"], 1, 1)
+                ys = np.stack([rng.integers(0, n, 64) for n in (s.b, s.ho, s.wo, s.kn)], 1)
+                ref = sampled.fwd_sampled(NHWC, s, c["w"].float().cpu().numpy(), c["x"].float().cpu().numpy(), ys)
+                got = sampled.nhwc_index(NHWC, c["y"].cpu().numpy(), ys)
+                e = _err(got, ref)
+                print(f"  {tag} problem {i} {p}: vs FP64 on the BF16 operands {e:.2e}", flush=True)
+                assert e <= TOL_DEQ[cg.BF16], (tag, i, p, e)
+
+        oracle_check("graph batch")
+        # the same problems in ONE persistent launch (static weights, graph): one kernel per run,
+        # each output bit-identical to the per-call run of the batch's plan
+        print("single-launch batch", flush=True)
+        b = cg.Batch([cg.problem(c["d"], c["w"].view(torch.int16), c["x"].view(torch.int16), c["y"])
+                      for c in cases], cg.BF16, cg.BATCH_GRAPH | cg.BATCH_STATIC_WEIGHTS | cg.BATCH_SINGLE)
+        assert b.info() == (1, 1, True), b.info()
+        refs = []
+        for i, c in enumerate(cases):
+            cfg, splits = b.plan(i)
+            r = torch.empty_like(c["y"])
+            cg.force_config(cfg, splits)
+            try:
+                cg.conv_ex(c["d"], c["w"].view(torch.int16), c["x"].view(torch.int16), r, dtype=cg.BF16)
+            finally:
+                cg.force_config(-1, 0)
+            refs.append(r)
+        torch.cuda.synchronize()
+        for rep in range(2):
+            for c in cases:
+                c["y"].fill_(float("nan"))
+            torch.cuda.synchronize()
+            b.run()
+            torch.cuda.synchronize()
+            for i, (c, r) in enumerate(zip(cases, refs)):
+                assert torch.equal(c["y"], r), (rep, i, c["p"], b.plan(i))
+        b.close()
+        oracle_check("single launch")
     finally:
         cg.set_planner(prev)
