@@ -221,40 +221,89 @@ __global__ void scale_kernel(float *out, long long n, float beta) {
     for (; i < n; i += stride) out[i] = beta == 0.0f ? 0.0f : beta * out[i];
 }
 
-// dst[z*tz + c*tc + r] = src[z*sz + r*sr + c] for r < R, c < Cc, z < gridDim.z:
-// a batch of 2-D transposes through a 32x33 LDS tile.
+// One 64x64 tile of a 2-D transpose, dst[c*tc + r] = src[r*sr + c] for
+// r0 <= r < min(r0 + 64, R), c0 <= c < min(c0 + 64, C), through LDS: 256
+// threads, each with its 16 loads in flight at once (64 consecutive elements
+// of a row per wave), then 16 stores along the transposed rows.  Every load
+// is issued (an element outside the tensor reads element 0 and is dropped):
+// a load guarded by a branch, with its LDS store right behind it, made the
+// compiler wait for each load in turn.  The row pitch of 65 keeps the column
+// reads on distinct banks.
+constexpr int TT = 64;
 template <typename E>
-__global__ void strided_transpose_kernel(const E *__restrict__ src, E *__restrict__ dst, int R, int Cc,
-                                         long long sz, long long sr, long long tz, long long tc) {
-    __shared__ E t[32][33];
-    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
-    const long long z = blockIdx.z;
-    for (int i = threadIdx.y; i < 32; i += blockDim.y) {
-        const int r = r0 + i, c = c0 + threadIdx.x;
-        if (r < R && c < Cc) t[i][threadIdx.x] = src[z * sz + (long long)r * sr + c];
+__device__ __attribute__((always_inline)) inline void transpose_tile(const E *__restrict__ src, E *__restrict__ dst,
+                                                                     int R, int C, long long sr, long long tc, int r0,
+                                                                     int c0, E (*t)[TT + 1]) {
+    const int tx = (int)threadIdx.x & (TT - 1), ty = (int)threadIdx.x >> 6;
+    const int c = c0 + tx;
+    E v[TT / 4];
+#pragma unroll
+    for (int i = 0; i < TT / 4; i++) {
+        const int r = r0 + 4 * i + ty;
+        v[i] = src[(r < R && c < C) ? (long long)r * sr + c : 0];
     }
+#pragma unroll
+    for (int i = 0; i < TT / 4; i++) t[4 * i + ty][tx] = v[i];
     __syncthreads();
-    for (int i = threadIdx.y; i < 32; i += blockDim.y) {
-        const int c = c0 + i, r = r0 + threadIdx.x;
-        if (r < R && c < Cc) dst[z * tz + (long long)c * tc + r] = t[threadIdx.x][i];
+    const int r = r0 + tx;
+#pragma unroll
+    for (int i = 0; i < TT; i += 4) {
+        const int cc = c0 + i + ty;
+        if (r < R && cc < C) dst[(long long)cc * tc + r] = t[tx][i + ty];
     }
 }
 
-// [d0][d1][d2] -> [d0][d2][d1] for a short d2 (a kernel's taps): one thread per
-// output element, d1 fastest, so the stores coalesce and the loads, d2
-// elements apart, share their lines through L2.  A 32x32 tile would leave
-// most of its threads idle on d2 = 9.
+// dst[z*tz + c*tc + r] = src[z*sz + r*sr + c] for r < R, c < Cc, z < gridDim.z:
+// a batch of 2-D transposes, one 64x64 tile per workgroup.
 template <typename E>
-__global__ void permute_short_kernel(const E *__restrict__ src, E *__restrict__ dst, int d1, int d2, long long n) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    const long long plane = (long long)d1 * d2;
-    for (; i < n; i += stride) {
-        const long long a = i / plane;
-        const int rem = (int)(i - a * plane);
-        const int t = rem / d1, c = rem - t * d1;  // output [a][t][c]
-        dst[i] = src[a * plane + (long long)c * d2 + t];
+__global__ void __launch_bounds__(256) strided_transpose_kernel(const E *__restrict__ src, E *__restrict__ dst, int R,
+                                                                int Cc, long long sz, long long sr, long long tz,
+                                                                long long tc) {
+    __shared__ E t[TT][TT + 1];
+    const long long z = blockIdx.z;
+    transpose_tile(src + z * sz, dst + z * tz, R, Cc, sr, tc, (int)blockIdx.y * TT, (int)blockIdx.x * TT, t);
+}
+
+// [d0][d1][d2] -> [d0][d2][d1] for a short d2 <= PS_D2 (a kernel's taps),
+// PS_ROWS rows of d1 at a time: the rows' PS_ROWS x d2 elements are one
+// contiguous run of the input, loaded into LDS with coalesced loads, then
+// stored as d2 runs of PS_ROWS consecutive elements of the output.  (A 64x64
+// tile would leave most of its threads idle on d2 = 9.)  Item `it` of a
+// permute with rb = ceil(d1 / PS_ROWS) row blocks per slice is slice it / rb.
+constexpr int PS_ROWS = 256, PS_D2 = 16;
+template <typename E>
+__device__ __attribute__((always_inline)) inline void permute_short_item(const E *__restrict__ src,
+                                                                         E *__restrict__ dst, int d1, int d2,
+                                                                         long long it, E *s) {
+    const int rb = (d1 + PS_ROWS - 1) / PS_ROWS;
+    const long long a = it / rb;
+    const int c0 = (int)(it - a * rb) * PS_ROWS;
+    const int nc = min(PS_ROWS, d1 - c0);
+    const long long slice = (long long)d1 * d2;
+    const E *sp = src + a * slice + (long long)c0 * d2;
+    E v[PS_D2];  // every load issued before the first LDS store (see transpose_tile)
+#pragma unroll
+    for (int k = 0; k < PS_D2; k++) {
+        const int i = (int)threadIdx.x + 256 * k;
+        v[k] = sp[i < nc * d2 ? i : 0];
     }
+#pragma unroll
+    for (int k = 0; k < PS_D2; k++) {
+        const int i = (int)threadIdx.x + 256 * k;
+        if (i < nc * d2) s[i] = v[k];
+    }
+    __syncthreads();
+    E *dp = dst + a * slice + c0;
+    const int c = (int)threadIdx.x;
+    if (c < nc)
+        for (int t = 0; t < d2; t++) dp[(long long)t * d1 + c] = s[c * d2 + t];
+}
+
+template <typename E>
+__global__ void __launch_bounds__(256) permute_short_kernel(const E *__restrict__ src, E *__restrict__ dst, int d1,
+                                                            int d2) {
+    __shared__ E s[PS_ROWS * PS_D2];
+    permute_short_item(src, dst, d1, d2, (long long)blockIdx.x, s);
 }
 
 // ---- a batch's element-wise jobs (cg_ewjob) in one launch.  A repack is a
@@ -279,17 +328,21 @@ __host__ __device__ inline EwGeom ew_geom(const cg_ewjob &j) {
     return g;
 }
 
+// kind 1 with a short d2 runs as permute_short_item
+__host__ __device__ inline bool ew_short(const cg_ewjob &j) { return j.kind == 1 && j.d2 <= PS_D2; }
+
 extern "C" long long cg_ew_items(const cg_ewjob *j) {
     if (j->kind >= 2) return (j->n + CG_EW_BLOCK - 1) / CG_EW_BLOCK;
+    if (ew_short(*j)) return (long long)j->d0 * ((j->d1 + PS_ROWS - 1) / PS_ROWS);
     const EwGeom g = ew_geom(*j);
-    return (long long)g.Z * ((g.R + 31) / 32) * ((g.C + 31) / 32);
+    return (long long)g.Z * ((g.R + TT - 1) / TT) * ((g.C + TT - 1) / TT);
 }
 
 template <typename E>
 __global__ void __launch_bounds__(256) ew_grouped_kernel(const cg_ewjob *__restrict__ jobs,
                                                          const int *__restrict__ start, int njobs, int total) {
-    __shared__ E t[32][33];
-    const int tx = (int)threadIdx.x & 31, ty = (int)threadIdx.x >> 5;
+    __shared__ E t[TT][TT + 1];
+    static_assert(PS_ROWS * PS_D2 <= TT * (TT + 1), "the short permute stages in the same LDS");
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         int lo = 0, hi = njobs - 1;
         while (lo < hi) {
@@ -318,22 +371,17 @@ __global__ void __launch_bounds__(256) ew_grouped_kernel(const cg_ewjob *__restr
             }
             continue;
         }
+        __syncthreads();  // the previous item's reads of the LDS are done
+        if (ew_short(j)) {
+            permute_short_item((const E *)j.src, (E *)j.dst, j.d1, j.d2, local, &t[0][0]);
+            continue;
+        }
         const EwGeom g = ew_geom(j);
-        const int tr = (g.R + 31) / 32, tcn = (g.C + 31) / 32;
+        const int tr = (g.R + TT - 1) / TT, tcn = (g.C + TT - 1) / TT;
         const int z = local / (tr * tcn), rem = local - z * (tr * tcn);
-        const int r0 = (rem / tcn) * 32, c0 = (rem - (rem / tcn) * tcn) * 32;
-        const E *src = (const E *)j.src + (long long)z * g.sz;
-        E *dst = (E *)j.dst + (long long)z * g.tz;
-        __syncthreads();  // the previous item's reads of t are done
-        for (int i = ty; i < 32; i += 8) {
-            const int r = r0 + i, c = c0 + tx;
-            if (r < g.R && c < g.C) t[i][tx] = src[(long long)r * g.sr + c];
-        }
-        __syncthreads();
-        for (int i = ty; i < 32; i += 8) {
-            const int c = c0 + i, r = r0 + tx;
-            if (r < g.R && c < g.C) dst[(long long)c * g.tc + r] = t[tx][i];
-        }
+        const int r0 = (rem / tcn) * TT, c0 = (rem - (rem / tcn) * tcn) * TT;
+        transpose_tile((const E *)j.src + (long long)z * g.sz, (E *)j.dst + (long long)z * g.tz, g.R, g.C, g.sr,
+                       g.tc, r0, c0, t);
     }
 }
 
@@ -525,9 +573,9 @@ extern "C" int cg_launch_scale(float *out, long long n, float beta, void *stream
 template <typename E>
 static int launch_transpose(const void *src, void *dst, int R, int Cc, int batch, long long sz, long long sr,
                             long long tz, long long tc, hipStream_t st) {
-    dim3 grid((Cc + 31) / 32, (R + 31) / 32, batch);
-    hipLaunchKernelGGL(strided_transpose_kernel<E>, grid, dim3(32, 8), 0, st, (const E *)src, (E *)dst, R, Cc, sz,
-                       sr, tz, tc);
+    dim3 grid((Cc + TT - 1) / TT, (R + TT - 1) / TT, batch);
+    hipLaunchKernelGGL(strided_transpose_kernel<E>, grid, dim3(256), 0, st, (const E *)src, (E *)dst, R, Cc, sz, sr,
+                       tz, tc);
     return (int)hipGetLastError();
 }
 
@@ -556,18 +604,17 @@ extern "C" int cg_launch_permute3(const void *src, void *dst, int d0, int d1, in
         R = d0; Cc = d2; batch = d1;
         sz = d2; sr = (long long)d1 * d2; tz = d0; tc = (long long)d1 * d0;
     } else {          // [d0][d1][d2] -> [d0][d2][d1]: batch over d0
-        if (d2 <= 16) {
-            const long long n = (long long)d0 * d1 * d2;
-            long long blocks = (n + 255) / 256;
-            if (blocks > 8192) blocks = 8192;
+        if (d2 <= PS_D2) {
+            const long long blocks = (long long)d0 * ((d1 + PS_ROWS - 1) / PS_ROWS);
+            if (blocks > 0x7fffffffLL) return (int)hipErrorInvalidValue;
             hipStream_t st = (hipStream_t)stream;
             switch (esize) {
             case 4: hipLaunchKernelGGL(permute_short_kernel<float>, dim3((unsigned)blocks), dim3(256), 0, st,
-                                       (const float *)src, (float *)dst, d1, d2, n); break;
+                                       (const float *)src, (float *)dst, d1, d2); break;
             case 2: hipLaunchKernelGGL(permute_short_kernel<uint16_t>, dim3((unsigned)blocks), dim3(256), 0, st,
-                                       (const uint16_t *)src, (uint16_t *)dst, d1, d2, n); break;
+                                       (const uint16_t *)src, (uint16_t *)dst, d1, d2); break;
             case 1: hipLaunchKernelGGL(permute_short_kernel<uint8_t>, dim3((unsigned)blocks), dim3(256), 0, st,
-                                       (const uint8_t *)src, (uint8_t *)dst, d1, d2, n); break;
+                                       (const uint8_t *)src, (uint8_t *)dst, d1, d2); break;
             default: return (int)hipErrorInvalidValue;
             }
             return (int)hipGetLastError();
@@ -588,7 +635,7 @@ extern "C" int cg_launch_permute3(const void *src, void *dst, int d0, int d1, in
 extern "C" int cg_launch_transpose2d(const void *src, long long lds, void *dst, long long ldd, int rows, int cols,
                                      int esize, void *stream) {
     if (rows <= 0 || cols <= 0) return 0;
-    if ((rows + 31) / 32 > 65535) return (int)hipErrorInvalidValue;
+    if ((rows + TT - 1) / TT > 65535) return (int)hipErrorInvalidValue;
     hipStream_t st = (hipStream_t)stream;
     switch (esize) {
     case 4: return launch_transpose<float>(src, dst, rows, cols, 1, 0, lds, 0, ldd, st);
