@@ -225,16 +225,24 @@ __device__ inline bool epi_staged(const cg_kparams &p) {
 
 // Stage the vectors of channels [n0, n0 + cnt) as 5 rows of cnt floats at s
 // (bias, mean, inv_std, gamma, bn_beta).  Called by all threads of the
-// workgroup between barriers.
+// workgroup between barriers.  The five loads of a channel are all issued
+// before its LDS stores: with each store right behind its own load, the
+// compiler waited for the loads one group at a time.
 __device__ inline void epi_stage(const cg_kparams &p, int n0, int cnt, float *s, int tid, int nthreads) {
     for (int i = tid; i < cnt; i += nthreads) {
         const int n = n0 + i;
-        const bool ok = n < p.KN;
-        s[i] = (p.bias && ok) ? p.bias[n] : 0.f;
-        s[cnt + i] = (p.mean && ok) ? p.mean[n] : 0.f;
-        s[2 * cnt + i] = (p.mean && ok) ? p.inv_std[n] : 1.f;
-        s[3 * cnt + i] = (p.mean && ok) ? p.gamma[n] : 1.f;
-        s[4 * cnt + i] = (p.mean && ok) ? p.bnb[n] : 0.f;
+        float v[5] = {0.f, 0.f, 1.f, 1.f, 0.f};
+        if (n < p.KN) {
+            if (p.bias) v[0] = p.bias[n];
+            if (p.mean) {
+                v[1] = p.mean[n];
+                v[2] = p.inv_std[n];
+                v[3] = p.gamma[n];
+                v[4] = p.bnb[n];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 5; k++) s[k * cnt + i] = v[k];
     }
 }
 
