@@ -678,6 +678,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tune-reps", type=int, default=3, help="timed repetitions per candidate when autotuning")
     ap.add_argument("--no-error", action="store_true", help="skip the FP64 error check")
     ap.add_argument("--no-tune", action="store_true", help="use the analytic planner instead of autotuning at setup")
     ap.add_argument("--no-precision", action="store_true", help="skip the BF16/FP8 precision sweep")
@@ -772,7 +773,7 @@ def main(argv=None):
     for cv in convs:
         if not a.no_tune and not cg.tune_get(cv["desc"]):
             d = cv["d"]
-            cg.autotune(cv["desc"], d["W"], d["X"], d["Y"], reps=3, stream=stream)
+            cg.autotune(cv["desc"], d["W"], d["X"], d["Y"], reps=a.tune_reps, stream=stream)
             tuned += 1
     if a.tune_db and tuned:
         cg.tune_save(a.tune_db)
