@@ -388,7 +388,7 @@ __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &
                 for (int v = 0; v < 16; v++) {
                     const int ml = wm * WTM + a * 32 + 8 * (v >> 2) + 4 * h + (v & 3), nl = wn * WTN + c * 32 + l32;
                     const int n = nt * BN + nl;
-                    if (ml < count && n < p.KN) store_out<CG_FWD_NHWC>(p, x, pbase + ml, n, acc[a][c][v]);
+                    if (ml < count && n < p.KN) store_out<CG_FWD_NHWC, (sizeof(T) < 4)>(p, x, pbase + ml, n, acc[a][c][v]);
                 }
         return;
     }
@@ -405,7 +405,7 @@ __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &
     for (int a = 0; a < TR; a++)
 #pragma unroll
         for (int c = 0; c < TC; c++)
-            store_block<CG_FWD_NHWC>(p, x, acc[a][c], s_epi, BN,
+            store_block<CG_FWD_NHWC, (sizeof(T) < 4)>(p, x, acc[a][c], s_epi, BN,
                                      [&](int v, int &m, int &n, int &nl) __attribute__((always_inline)) {
                                          const int ml = wm * WTM + a * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
                                          nl = wn * WTN + c * 32 + l32;

@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(256) sk_fixup_kernel(const cg_kparams p, int l
                     }
                 }
             }
-            p.out[out_addr<OP>(p, x, m, n)] = epi_value<OP>(p, acc[j], old[j], ev);
+            put_out<false>(p.out, out_addr<OP>(p, x, m, n), epi_value<OP>(p, acc[j], old[j], ev));
         }
     }
 }

@@ -695,7 +695,7 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
                     const int ic = coff + b * 32 + (lane & 31);
                     const int ml = SW ? ic : ir, nl = SW ? ir : ic;
                     const int m = mt * BM + ml, n = nt * BN + nl;
-                    if (m < x.Mrows && n < p.KN) store_out<OP>(p, x, m, n, acc[a][b][v]);
+                    if (m < x.Mrows && n < p.KN) store_out<OP, (sizeof(T) < 4)>(p, x, m, n, acc[a][b][v]);
                 }
         return;
     }
@@ -712,7 +712,7 @@ __device__ __attribute__((always_inline)) inline void igemm_tile(const cg_kparam
     for (int a = 0; a < TR; a++)
 #pragma unroll
         for (int b = 0; b < TC; b++)
-            store_block<OP>(p, x, acc[a][b], s_epi, BN, [&](int v, int &m, int &n, int &nl) __attribute__((always_inline)) {
+            store_block<OP, (sizeof(T) < 4)>(p, x, acc[a][b], s_epi, BN, [&](int v, int &m, int &n, int &nl) __attribute__((always_inline)) {
                 const int ir = roff + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
                 const int ic = coff + b * 32 + (lane & 31);
                 const int ml = SW ? ic : ir;

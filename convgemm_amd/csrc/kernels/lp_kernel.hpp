@@ -397,7 +397,7 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
                 for (int v = 0; v < 16; v++) {
                     const int ml = wm * WTM + a * 32 + 8 * (v >> 2) + 4 * h + (v & 3), nl = wn * WTN + b * 32 + l32;
                     const int m = mt * BM + ml, n = nt * BN + nl;
-                    if (m < x.Mrows && n < p.KN) store_out<CG_FWD_NHWC>(p, x, m, n, acc[a][b][v]);
+                    if (m < x.Mrows && n < p.KN) store_out<CG_FWD_NHWC, (ES < 4)>(p, x, m, n, acc[a][b][v]);
                 }
         return;
     }
@@ -414,7 +414,7 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
     for (int a = 0; a < TR; a++)
 #pragma unroll
         for (int b = 0; b < TC; b++)
-            store_block<CG_FWD_NHWC>(p, x, acc[a][b], s_epi, BN, [&](int v, int &m, int &n, int &nl) __attribute__((always_inline)) {
+            store_block<CG_FWD_NHWC, (ES < 4)>(p, x, acc[a][b], s_epi, BN, [&](int v, int &m, int &n, int &nl) __attribute__((always_inline)) {
                 const int ml = wm * WTM + a * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
                 nl = wn * WTN + b * 32 + l32;
                 m = mt * BM + ml;
