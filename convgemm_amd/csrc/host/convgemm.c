@@ -560,6 +560,16 @@ static int weight_repack(const convgemm_desc_t *d, int elem, int *d0, int *d1, i
     return 0;
 }
 
+/* Can the plan read the caller's weights in place (cg_kparams.braw)?  The
+ * pipelined forward NHWC kernel (FP32, BF16) reads [c][kh][kw][kn] rows of
+ * whole 16-byte kn chunks from a 16-byte aligned tensor; weights already
+ * prepacked, and the backward-data forward form (whose weights need their
+ * taps reversed), keep the repacked order. */
+static int raw_weights_ok(int elem, const cg_kparams *kp, const cg_tilecfg *t, const void *in) {
+    return kp->op == CG_FWD_NHWC && !kp->ffd && kp->vec && t->lp > 0 && !t->halo && elem != CG_ELEM_FP8 &&
+           (kp->KN * esize_of(elem)) % 16 == 0 && (uintptr_t)in % 16 == 0;
+}
+
 int cg_weight_repack(const convgemm_desc_t *d, int elem, int *d0, int *d1, int *d2, int *perm) {
     return weight_repack(d, elem, d0, d1, d2, perm);
 }
@@ -762,10 +772,7 @@ static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, 
 
     /* weights in the order the kernels read */
     int rd0 = 0, rd1 = 0, rd2 = 0, rperm = 0;
-    const int repack = weight_repack(d, elem, &rd0, &rd1, &rd2, &rperm) && !ws->prepacked;
-    const size_t wt_bytes = repack ? (size_t)K * d->kn * es : 0;
-    if (repack && ws->ws_w && ws->w_bytes < wt_bytes)
-        return set_error(CONVGEMM_ERR_WORKSPACE, "weight workspace %zu < %zu bytes", ws->w_bytes, wt_bytes);
+    int repack = weight_repack(d, elem, &rd0, &rd1, &rd2, &rperm) && !ws->prepacked;
 
     /* operand roles: A = X (FWD, WGRAD) or dY (DGRAD); B = weights (FWD,
      * DGRAD) or dY (WGRAD) */
@@ -788,6 +795,14 @@ static convgemm_status_t run(const convgemm_desc_t *d, float alpha, float beta, 
     /* splits == 0: stream-K (forward and weight gradient) */
     kp.sk = splits == 0 ? cg_sk_split(elem, cfg, &kp, &kp.sk_dp) : 0;
     if (!kp.sk) kp.sk_dp = 0;
+    /* a pipelined forward plan reads the caller's weights in place: no repack */
+    if (repack && raw_weights_ok(elem, &kp, &t, in)) {
+        repack = 0;
+        kp.braw = 1;
+    }
+    const size_t wt_bytes = repack ? (size_t)K * d->kn * es : 0;
+    if (repack && ws->ws_w && ws->w_bytes < wt_bytes)
+        return set_error(CONVGEMM_ERR_WORKSPACE, "weight workspace %zu < %zu bytes", ws->w_bytes, wt_bytes);
 
     /* workspace: repacked weights + split-K partials / stream-K pieces */
     size_t pbytes = kp.sk ? cg_sk_bytes(kp.sk, &t) : cg_partial_bytes(&kp);

@@ -65,7 +65,22 @@ __device__ __attribute__((always_inline)) inline void step_next(const cg_kparams
 // EXT: the NS stages live at ext + s * STAGE (a kernel that runs several tile
 // shapes gives them one LDS buffer of the largest size); otherwise in LDS
 // objects of the tile's own.
-template <typename T, int BM, int BN, int WM, int WN, int NS, bool EXT = false>
+// RAWB (FP32, BF16): p.Bm holds the caller's weights, [c][kh][kw][kn], read in
+// place.  A stage's weight tile is then KSW rows, one per channel of the
+// step, each holding the tile's BN kn (chunk c of row r at c ^ rw_swz(r));
+// the MFMA reads it transposed: BF16 with ds_read_b64_tr_b16 (lane j of a
+// 16-lane group gets column m0 + j at 4 consecutive rows, as in the weight-
+// gradient kernel), FP32 one 4-byte element per MFMA.  The K order is the
+// activations': lane half h, group g, element e is channel 16g + 8h + e
+// (BF16) or 8g + 4h + e (FP32) of the step.
+template <int ES, int CPW>
+__device__ __forceinline__ int rw_swz(int r) {
+    if constexpr (ES == 4) return ((r >> 2) & 1) << 3;  // rows 8g + j and 8g + 4 + j on distinct bank halves
+    else if constexpr (CPW >= 16) return (r & 3) << 2;
+    else return ((r >> 1) & 1) << 2;
+}
+
+template <typename T, int BM, int BN, int WM, int WN, int NS, bool EXT = false, bool RAWB = false>
 __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &p, const int lin, const int sk_k0 = -1,
                                                          const int sk_k1 = 0, float *piece = nullptr,
                                                          char *ext = nullptr) {
@@ -125,14 +140,32 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
     }
     int b_off[IB], b_cb[IB];
     bool b_ok[IB];
+    constexpr int KSW = RB / ES;   // channels per K step
+    constexpr int RWB = BN * ES;   // RAWB: bytes of one weight row (one channel's BN kn)
+    constexpr int CPW = RWB / 16;  // RAWB: 16-byte chunks per weight row
+    constexpr int PIW = 64 / CPW;  // RAWB: weight rows per load instruction
+    static_assert(!RAWB || (ES != 1 && CPW >= (ES == 4 ? 16 : 8) && 64 % CPW == 0 && KSW / (PIW * NW) == IB &&
+                            KSW % (PIW * NW) == 0), "raw weight rows");
+    if constexpr (RAWB) {  // row r = channel (step base + r), b_off = its kn chunk's byte offset without the step's
 #pragma unroll
-    for (int j = 0; j < IB; j++) {
-        const int r = (wave * IB + j) * 8 + lrow;
-        const int chunk = lpos ^ ((r >> 1) & 7);
-        b_cb[j] = chunk * 16;
-        const int n = nt * BN + r;
-        b_ok[j] = n < p.KN;
-        b_off[j] = n * p.taps * CB + b_cb[j];
+        for (int j = 0; j < IB; j++) {
+            const int r = (wave * IB + j) * PIW + lane / CPW;
+            const int chunk = (lane % CPW) ^ rw_swz<ES, CPW>(r);
+            const int n = nt * BN + chunk * (16 / ES);
+            b_cb[j] = r;
+            b_ok[j] = n < p.KN;
+            b_off[j] = (int)(((unsigned)r * (unsigned)p.taps * (unsigned)p.KN + (unsigned)n) * ES);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < IB; j++) {
+            const int r = (wave * IB + j) * 8 + lrow;
+            const int chunk = lpos ^ ((r >> 1) & 7);
+            b_cb[j] = chunk * 16;
+            const int n = nt * BN + r;
+            b_ok[j] = n < p.KN;
+            b_off[j] = n * p.taps * CB + b_cb[j];
+        }
     }
 
     // ---- reduction range (split-K) ----
@@ -188,8 +221,17 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
                     rA, (lds_void_t *)(uintptr_t)(base + (wave * IA + j) * 1024), 16, off, 0, 0, 0);
             } else {
                 const int j = jj - IA;
-                const bool ok = live && b_ok[j] && b_cb[j] < lim;
-                const unsigned off = ok ? (unsigned)(b_off[j] + tapB) : OOB;
+                bool ok;
+                unsigned off;
+                if constexpr (RAWB) {  // rows past C read 0
+                    ok = live && b_ok[j] && b_cb[j] < p.C - i_cb * KSW;
+                    off = ok ? (unsigned)b_off[j] +
+                                   ((unsigned)(i_cb * KSW) * (unsigned)p.taps + (unsigned)i_tap) * (unsigned)p.KN * ES
+                             : OOB;
+                } else {
+                    ok = live && b_ok[j] && b_cb[j] < lim;
+                    off = ok ? (unsigned)(b_off[j] + tapB) : OOB;
+                }
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(
                     rB, (lds_void_t *)(uintptr_t)(base + BM * RB + (wave * IB + j) * 1024), 16, off, 0, 0, 0);
             }
@@ -227,14 +269,51 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
     struct Ops {
         OpV a[TR], b[TC];
     };
+    // RAWB: per-lane byte offset of this lane's weight elements in each 32-column block
+    int woff[TC];
+    if constexpr (RAWB && ES == 2) {
+        const int G = lane >> 4, s16 = lane & 15;
+        const int kl = 8 * (G >> 1) + (s16 >> 2);  // first row of the group's first read
+        const int ml = 16 * (G & 1) + 4 * (s16 & 3);
+        const int sw = rw_swz<ES, CPW>(kl);       // == that of kl + 4 and kl + 16g
+#pragma unroll
+        for (int b = 0; b < TC; b++) {
+            const int n = wn * WTN + b * 32 + ml;
+            woff[b] = BM * RB + kl * RWB + (((n >> 3) ^ sw) << 4) + (n & 7) * 2;
+        }
+    } else if constexpr (RAWB) {
+#pragma unroll
+        for (int b = 0; b < TC; b++) {
+            const int n = wn * WTN + b * 32 + l32;
+            woff[b] = BM * RB + 4 * h * RWB + (((n >> 2) ^ (h << 3)) << 4) + (n & 3) * 4;
+        }
+    }
     auto ldg = [&](auto s, int g, Ops &r) __attribute__((always_inline)) {
         const char *base = stage_ptr(s);
         if constexpr (ES != 1) {
             const int co = ((2 * g + h) ^ swz) * 16;
 #pragma unroll
             for (int a = 0; a < TR; a++) r.a[a] = *(const i32x4 *)(base + rowA + a * 32 * RB + co);
+            if constexpr (RAWB && ES == 2) {
+                auto tr = [&](int off) __attribute__((always_inline)) -> i32x2 {
+                    typedef short s16x4 __attribute__((ext_vector_type(4)));
+                    typedef __attribute__((address_space(3))) s16x4 lds_s16x4;
+                    return __builtin_bit_cast(i32x2, __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4 *)(base + off)));
+                };
 #pragma unroll
-            for (int b = 0; b < TC; b++) r.b[b] = *(const i32x4 *)(base + rowB + b * 32 * RB + co);
+                for (int b = 0; b < TC; b++) {
+                    const i32x2 lo = tr(woff[b] + 16 * g * RWB), hi = tr(woff[b] + (16 * g + 4) * RWB);
+                    r.b[b] = i32x4{lo[0], lo[1], hi[0], hi[1]};
+                }
+            } else if constexpr (RAWB) {
+#pragma unroll
+                for (int b = 0; b < TC; b++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) r.b[b][j] = *(const int *)(base + woff[b] + (8 * g + j) * RWB);
+            } else {
+#pragma unroll
+                for (int b = 0; b < TC; b++) r.b[b] = *(const i32x4 *)(base + rowB + b * 32 * RB + co);
+            }
         } else {
             const int c0 = 4 * g + 2 * h;
             const int co0 = (c0 ^ swz) * 16, co1 = ((c0 + 1) ^ swz) * 16;
@@ -424,9 +503,9 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
     if (s_epi) __syncthreads();  // the LDS is rewritten by the next item of a persistent launch
 }
 
-template <typename T, int BM, int BN, int WM, int WN, int NS>
+template <typename T, int BM, int BN, int WM, int WN, int NS, bool RAWB>
 __global__ void __launch_bounds__(WM * WN * 64) lp_kernel(const cg_kparams p) {
-    lp_tile<T, BM, BN, WM, WN, NS>(p, xcd_run(blockIdx.x, gridDim.x));
+    lp_tile<T, BM, BN, WM, WN, NS, false, RAWB>(p, xcd_run(blockIdx.x, gridDim.x));
 }
 
 // Stream-K: blocks 0 .. sk_dp-1 each run one whole tile of the first sk_dp
@@ -443,12 +522,12 @@ __global__ void __launch_bounds__(WM * WN * 64) lp_kernel(const cg_kparams p) {
 // entry, the scalar state live across the body overflowed the SGPRs and the
 // spills were reloaded with v_readlane all through the K loop (2580 reloads
 // in the 64x128 NCHW kernel, against 21 now).
-template <typename T, int BM, int BN, int WM, int WN, int NS>
+template <typename T, int BM, int BN, int WM, int WN, int NS, bool RAWB>
 __global__ void __launch_bounds__(WM * WN * 64) lp_sk_kernel(const cg_kparams p_arg) {
     const cg_kparams &p = kernarg_params(p_arg);
     SkRange r = sk_range(p);
     for (SkPiece q; sk_next(p, r, q, BM * BN);) {
-        lp_tile<T, BM, BN, WM, WN, NS>(p, q.tile, q.k0, q.k1, q.piece);
+        lp_tile<T, BM, BN, WM, WN, NS, false, RAWB>(p, q.tile, q.k0, q.k1, q.piece);
         __syncthreads();  // loads, LDS reads and stores of this piece are complete
     }
 }
@@ -472,13 +551,15 @@ __global__ void __launch_bounds__(WM * WN * 64) lp_grouped_kernel(const cg_kpara
     }
 }
 
-// Forward NHWC with 16-byte channel chunks only.
+// Forward NHWC with 16-byte channel chunks only.  kp->braw: the weights are
+// the caller's [c][kh][kw][kn] (FP32, BF16; single and stream-K launches).
 template <typename T, int BM, int BN, int WM, int WN, int NS>
 inline int launch_lp(const cg_kparams *kp, const cg_group *g, int *occ, hipStream_t st) {
     constexpr int NT = WM * WN * 64;
     // FP8 wave tiles of 128x64 and up hold 128 accumulator registers beside
     // 32-byte operands: their stream-K form spills, and is not built
     constexpr bool kSK = !(sizeof(T) == 1 && (BM / WM) * (BN / WN) >= 128 * 64);
+    constexpr bool kRaw = sizeof(T) != 1;
     if (kp->op != CG_FWD_NHWC || !kp->vec) return (int)hipErrorInvalidValue;
     if (occ) {
         if (kp->sk < 0) {
@@ -487,18 +568,25 @@ inline int launch_lp(const cg_kparams *kp, const cg_group *g, int *occ, hipStrea
                 return 0;
             } else {
                 return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    occ, reinterpret_cast<const void *>(&lp_sk_kernel<T, BM, BN, WM, WN, NS>), NT, 0);
+                    occ, reinterpret_cast<const void *>(&lp_sk_kernel<T, BM, BN, WM, WN, NS, false>), NT, 0);
             }
         }
         return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(
             occ, reinterpret_cast<const void *>(&lp_grouped_kernel<T, BM, BN, WM, WN, NS>), NT, 0);
     }
+    if (kp->braw && (g || !kRaw)) return (int)hipErrorInvalidValue;  // grouped launches read repacked weights
     if (kp->sk > 0 && !g) {
         if constexpr (!kSK) {
             return (int)hipErrorInvalidValue;
         } else {
-            hipLaunchKernelGGL((lp_sk_kernel<T, BM, BN, WM, WN, NS>), dim3((unsigned)(kp->sk_dp + kp->sk)), dim3(NT),
-                               0, st, *kp);
+            const dim3 grid((unsigned)(kp->sk_dp + kp->sk));
+            if constexpr (kRaw) {
+                if (kp->braw) {
+                    hipLaunchKernelGGL((lp_sk_kernel<T, BM, BN, WM, WN, NS, true>), grid, dim3(NT), 0, st, *kp);
+                    return (int)hipGetLastError();
+                }
+            }
+            hipLaunchKernelGGL((lp_sk_kernel<T, BM, BN, WM, WN, NS, false>), grid, dim3(NT), 0, st, *kp);
             return (int)hipGetLastError();
         }
     }
@@ -509,7 +597,13 @@ inline int launch_lp(const cg_kparams *kp, const cg_group *g, int *occ, hipStrea
         return (int)hipGetLastError();
     }
     const long long nb = (long long)kp->mtiles * kp->ntiles * kp->splits;
-    hipLaunchKernelGGL((lp_kernel<T, BM, BN, WM, WN, NS>), dim3((unsigned)nb), dim3(NT), 0, st, *kp);
+    if constexpr (kRaw) {
+        if (kp->braw) {
+            hipLaunchKernelGGL((lp_kernel<T, BM, BN, WM, WN, NS, true>), dim3((unsigned)nb), dim3(NT), 0, st, *kp);
+            return (int)hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((lp_kernel<T, BM, BN, WM, WN, NS, false>), dim3((unsigned)nb), dim3(NT), 0, st, *kp);
     return (int)hipGetLastError();
 }
 
