@@ -5,10 +5,13 @@
 // [start[j], start[j+1]) (its tiles, padded to a multiple of 8) and runs on
 // tile shape var[j] of SingleSet<T> (tile_tables.hpp), the pipelined tile of
 // lp_kernel.hpp with its stages in one LDS buffer that every shape shares.
+// The weights are read repacked ([kn][kh][kw][c], the batch's element-wise
+// launch or its static repack): reading the caller's order in place made
+// this kernel 1.5x slower on the 256-problem BF16 batch.
 // Every shape holds 4 waves and at most SINGLE_LDS bytes of stages, so two
 // workgroups are resident per CU whatever mix of shapes they run.
 //
-// The host orders the problems by K steps per tile, longest first, and
+// The host orders the problems by the MFMA work of a tile, longest first, and
 // launches as many workgroups as are resident at once.  Round r hands item
 // r*G + w to workgroup slot w; odd rounds reverse the slots in groups of 8,
 // so the longest-first order evens out across workgroups while an item keeps
