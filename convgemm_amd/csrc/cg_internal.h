@@ -76,7 +76,7 @@ typedef struct {
     int fds_howo, fds_wo, fds_c, fds_hp;
     int ffd;         /* 1: the forward form of a unit-stride NHWC backward-data problem (op is CG_FWD_NHWC;
                         only the planner's problem key tells the two apart) */
-    int braw;        /* FWD NHWC on a pipelined (LP) configuration, FP32/BF16: Bm holds the caller's
+    int braw;        /* FWD NHWC on a pipelined (LP) configuration: Bm holds the caller's
                         [c][kh][kw][kn] weights, read in place (no repack); 0 otherwise */
 } cg_kparams;
 

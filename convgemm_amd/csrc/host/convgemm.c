@@ -561,20 +561,24 @@ static int weight_repack(const convgemm_desc_t *d, int elem, int *d0, int *d1, i
 }
 
 /* Should the plan read the caller's weights in place (cg_kparams.braw)?  The
- * pipelined forward NHWC kernel (FP32, BF16) can read [c][kh][kw][kn] rows of
- * whole 16-byte kn chunks from a 16-byte aligned tensor; weights already
+ * pipelined forward NHWC kernel can read [c][kh][kw][kn] rows of whole
+ * 16-byte kn chunks from a 16-byte aligned tensor; weights already
  * prepacked, and the backward-data forward form (whose weights need their
  * taps reversed), keep the repacked order.  Reading in place saves the
  * repack launch (about 4.5 us) but the kernel then reads its weight operand
  * with transposing LDS reads: measured on the same plans it ran 2-18%
  * (mean 8%) longer in BF16 and 0-5% (mean 3.5%) in FP32, so it pays below
- * roughly 24 (BF16) or 10 (FP32) GFLOP per call. */
+ * roughly 24 (BF16) or 10 (FP32) GFLOP per call.  In FP8 the best plan of a
+ * layer ran between 15% shorter and 7% longer than the best repacked one
+ * (the repacked FP8 operand reads make the compiler spill on the larger
+ * tiles), and reading in place paid or tied up to the largest layer
+ * measured, 118 GFLOP. */
 static int raw_weights_ok(int elem, const cg_kparams *kp, const cg_tilecfg *t, const void *in) {
-    if (kp->op != CG_FWD_NHWC || kp->ffd || !kp->vec || t->lp <= 0 || t->halo || elem == CG_ELEM_FP8 ||
+    if (kp->op != CG_FWD_NHWC || kp->ffd || !kp->vec || t->lp <= 0 || t->halo ||
         (kp->KN * esize_of(elem)) % 16 || (uintptr_t)in % 16)
         return 0;
     const double flops = 2.0 * (double)kp->P * kp->C * kp->taps * kp->KN;
-    return flops <= (elem == CG_ELEM_BF16 ? 24e9 : 10e9);
+    return flops <= (elem == CG_ELEM_FP8 ? 100e9 : elem == CG_ELEM_BF16 ? 24e9 : 10e9);
 }
 
 int cg_weight_repack(const convgemm_desc_t *d, int elem, int *d0, int *d1, int *d2, int *perm) {

@@ -65,17 +65,22 @@ __device__ __attribute__((always_inline)) inline void step_next(const cg_kparams
 // EXT: the NS stages live at ext + s * STAGE (a kernel that runs several tile
 // shapes gives them one LDS buffer of the largest size); otherwise in LDS
 // objects of the tile's own.
-// RAWB (FP32, BF16): p.Bm holds the caller's weights, [c][kh][kw][kn], read in
-// place.  A stage's weight tile is then KSW rows, one per channel of the
-// step, each holding the tile's BN kn (chunk c of row r at c ^ rw_swz(r));
-// the MFMA reads it transposed: BF16 with ds_read_b64_tr_b16 (lane j of a
-// 16-lane group gets column m0 + j at 4 consecutive rows, as in the weight-
-// gradient kernel), FP32 one 4-byte element per MFMA.  The K order is the
-// activations': lane half h, group g, element e is channel 16g + 8h + e
-// (BF16) or 8g + 4h + e (FP32) of the step.
+// RAWB: p.Bm holds the caller's weights, [c][kh][kw][kn], read in place.  A
+// stage's weight tile is then KSW rows, one per channel of the step, each
+// holding the tile's BN kn (chunk c of row r at c ^ rw_swz(r)); the MFMA
+// reads it transposed: BF16 with ds_read_b64_tr_b16 (lane j of a 16-lane
+// group gets column m0 + j at 4 consecutive rows, as in the weight-gradient
+// kernel), FP8 with ds_read_b64_tr_b8 (lane j of a 16-lane group gets
+// column m0 + j at 8 consecutive rows: lanes 2e and 2e + 1 address row e,
+// columns m0..m0+7 and m0+8..m0+15), FP32 one 4-byte element per MFMA.  The
+// K order is the activations': lane half h, group g, element e is channel
+// 16g + 8h + e (BF16), 64g + 32h + e (FP8) or 8g + 4h + e (FP32) of the step.
+// The FP8 swizzles put the 8 rows x 16 bytes of one 16-lane read on 32
+// distinct banks, and the two chunks of a half-wave on 64.
 template <int ES, int CPW>
 __device__ __forceinline__ int rw_swz(int r) {
     if constexpr (ES == 4) return ((r >> 2) & 1) << 3;  // rows 8g + j and 8g + 4 + j on distinct bank halves
+    else if constexpr (ES == 1) return CPW >= 8 ? (r & 7) : ((r >> 1) & 3);
     else if constexpr (CPW >= 16) return (r & 3) << 2;
     else return ((r >> 1) & 1) << 2;
 }
@@ -144,7 +149,7 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
     constexpr int RWB = BN * ES;   // RAWB: bytes of one weight row (one channel's BN kn)
     constexpr int CPW = RWB / 16;  // RAWB: 16-byte chunks per weight row
     constexpr int PIW = 64 / CPW;  // RAWB: weight rows per load instruction
-    static_assert(!RAWB || (ES != 1 && CPW >= (ES == 4 ? 16 : 8) && 64 % CPW == 0 && KSW / (PIW * NW) == IB &&
+    static_assert(!RAWB || (CPW >= (ES == 4 ? 16 : ES == 2 ? 8 : 4) && 64 % CPW == 0 && KSW / (PIW * NW) == IB &&
                             KSW % (PIW * NW) == 0), "raw weight rows");
     if constexpr (RAWB) {  // row r = channel (step base + r), b_off = its kn chunk's byte offset without the step's
 #pragma unroll
@@ -281,6 +286,15 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
             const int n = wn * WTN + b * 32 + ml;
             woff[b] = BM * RB + kl * RWB + (((n >> 3) ^ sw) << 4) + (n & 7) * 2;
         }
+    } else if constexpr (RAWB && ES == 1) {
+        const int G = lane >> 4, s16 = lane & 15;
+        const int kl = 32 * (G >> 1) + (s16 >> 1);  // row of read 0 of group 0
+        const int sw = rw_swz<ES, CPW>(kl);         // == that of kl + 64g + 8q
+#pragma unroll
+        for (int b = 0; b < TC; b++) {
+            const int c = ((wn * WTN + b * 32) >> 4) + (G & 1);
+            woff[b] = BM * RB + kl * RWB + ((c ^ sw) << 4) + 8 * (s16 & 1);
+        }
     } else if constexpr (RAWB) {
 #pragma unroll
         for (int b = 0; b < TC; b++) {
@@ -323,11 +337,24 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
                 const i32x4 hi = *(const i32x4 *)(base + rowA + a * 32 * RB + co1);
                 r.a[a] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
             }
+            if constexpr (RAWB) {
+                auto tr = [&](int off) __attribute__((always_inline)) -> i32x2 {
+                    typedef __attribute__((address_space(3))) i32x2 lds_i32x2;
+                    return __builtin_amdgcn_ds_read_tr8_b64_v2i32((lds_i32x2 *)(base + off));
+                };
 #pragma unroll
-            for (int b = 0; b < TC; b++) {
-                const i32x4 lo = *(const i32x4 *)(base + rowB + b * 32 * RB + co0);
-                const i32x4 hi = *(const i32x4 *)(base + rowB + b * 32 * RB + co1);
-                r.b[b] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+                for (int b = 0; b < TC; b++) {
+                    const int o = woff[b] + 64 * g * RWB;
+                    const i32x2 q0 = tr(o), q1 = tr(o + 8 * RWB), q2 = tr(o + 16 * RWB), q3 = tr(o + 24 * RWB);
+                    r.b[b] = i32x8{q0[0], q0[1], q1[0], q1[1], q2[0], q2[1], q3[0], q3[1]};
+                }
+            } else {
+#pragma unroll
+                for (int b = 0; b < TC; b++) {
+                    const i32x4 lo = *(const i32x4 *)(base + rowB + b * 32 * RB + co0);
+                    const i32x4 hi = *(const i32x4 *)(base + rowB + b * 32 * RB + co1);
+                    r.b[b] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+                }
             }
         }
     };
@@ -552,14 +579,13 @@ __global__ void __launch_bounds__(WM * WN * 64) lp_grouped_kernel(const cg_kpara
 }
 
 // Forward NHWC with 16-byte channel chunks only.  kp->braw: the weights are
-// the caller's [c][kh][kw][kn] (FP32, BF16; single and stream-K launches).
+// the caller's [c][kh][kw][kn] (plain and stream-K launches).
 template <typename T, int BM, int BN, int WM, int WN, int NS>
 inline int launch_lp(const cg_kparams *kp, const cg_group *g, int *occ, hipStream_t st) {
     constexpr int NT = WM * WN * 64;
     // FP8 wave tiles of 128x64 and up hold 128 accumulator registers beside
     // 32-byte operands: their stream-K form spills, and is not built
     constexpr bool kSK = !(sizeof(T) == 1 && (BM / WM) * (BN / WN) >= 128 * 64);
-    constexpr bool kRaw = sizeof(T) != 1;
     if (kp->op != CG_FWD_NHWC || !kp->vec) return (int)hipErrorInvalidValue;
     if (occ) {
         if (kp->sk < 0) {
@@ -574,17 +600,15 @@ inline int launch_lp(const cg_kparams *kp, const cg_group *g, int *occ, hipStrea
         return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(
             occ, reinterpret_cast<const void *>(&lp_grouped_kernel<T, BM, BN, WM, WN, NS>), NT, 0);
     }
-    if (kp->braw && (g || !kRaw)) return (int)hipErrorInvalidValue;  // grouped launches read repacked weights
+    if (kp->braw && g) return (int)hipErrorInvalidValue;  // grouped launches read repacked weights
     if (kp->sk > 0 && !g) {
         if constexpr (!kSK) {
             return (int)hipErrorInvalidValue;
         } else {
             const dim3 grid((unsigned)(kp->sk_dp + kp->sk));
-            if constexpr (kRaw) {
-                if (kp->braw) {
-                    hipLaunchKernelGGL((lp_sk_kernel<T, BM, BN, WM, WN, NS, true>), grid, dim3(NT), 0, st, *kp);
-                    return (int)hipGetLastError();
-                }
+            if (kp->braw) {
+                hipLaunchKernelGGL((lp_sk_kernel<T, BM, BN, WM, WN, NS, true>), grid, dim3(NT), 0, st, *kp);
+                return (int)hipGetLastError();
             }
             hipLaunchKernelGGL((lp_sk_kernel<T, BM, BN, WM, WN, NS, false>), grid, dim3(NT), 0, st, *kp);
             return (int)hipGetLastError();
@@ -597,11 +621,9 @@ inline int launch_lp(const cg_kparams *kp, const cg_group *g, int *occ, hipStrea
         return (int)hipGetLastError();
     }
     const long long nb = (long long)kp->mtiles * kp->ntiles * kp->splits;
-    if constexpr (kRaw) {
-        if (kp->braw) {
-            hipLaunchKernelGGL((lp_kernel<T, BM, BN, WM, WN, NS, true>), dim3((unsigned)nb), dim3(NT), 0, st, *kp);
-            return (int)hipGetLastError();
-        }
+    if (kp->braw) {
+        hipLaunchKernelGGL((lp_kernel<T, BM, BN, WM, WN, NS, true>), dim3((unsigned)nb), dim3(NT), 0, st, *kp);
+        return (int)hipGetLastError();
     }
     hipLaunchKernelGGL((lp_kernel<T, BM, BN, WM, WN, NS, false>), dim3((unsigned)nb), dim3(NT), 0, st, *kp);
     return (int)hipGetLastError();

@@ -500,9 +500,56 @@ def test_prepacked_weights_bit_identical(dtype):
         cg.conv_packed(d, packed, Xin, y1, beta=beta, dtype=elem, scales=scales)
         torch.cuda.synchronize()
         kplan = cg.last_plan()
-        # a pipelined forward NHWC plan (FP32, BF16) reads the caller's weights in place: conv_ex
-        # runs no repack either, and the kernel reads the other weight order with the same MFMA inputs
+        # a pipelined forward NHWC plan whose KN fills 16-byte chunks reads the caller's weights in
+        # place: conv_ex runs no repack either, and the kernel reads the other weight order with the
+        # same MFMA inputs
         t = cg.tile_configs(elem)[plan[0]]
-        raw = op == cg.FWD and layout == NHWC and dtype != "fp8" and t[5] > 0 and t[7] == 0
+        es = {"f32": 4, "bf16": 2, "fp8": 1}[dtype]
+        raw = op == cg.FWD and layout == NHWC and t[5] > 0 and t[7] == 0 and (s.kn * es) % 16 == 0
         assert kplan[:2] == plan[:2] and kplan[2] == plan[2] - (1 if n and not raw else 0), (plan, kplan, raw)
         assert torch.equal(y0, y1), (op, layout)
+
+
+INPLACE_SHAPES = [
+    Shape(2, 12, 11, 64, 80, 3, 3, 1, 1, 2, 1, 1, 1),   # N tail, stride
+    Shape(1, 9, 9, 160, 48, 3, 3, 1, 1, 1, 1, 1, 1),    # ragged last channel block
+    Shape(3, 7, 6, 32, 16, 1, 1, 0, 0, 1, 1, 1, 1),     # M tail, one K step
+]
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "fp8"])
+def test_inplace_weights_every_pipelined_config(dtype):
+    """Every pipelined forward NHWC configuration, split-K 1 and 2: conv_ex reads the
+    caller's [c][kh][kw][kn] weights in place (transposing LDS reads, no repack kernel)
+    and gives the bits of conv_packed, which reads them repacked to [kn][kh][kw][c]."""
+    elem = {"f32": cg.F32, "bf16": cg.BF16, "fp8": cg.FP8_E4M3}[dtype]
+    tdt = {"f32": torch.float32, "bf16": torch.bfloat16, "fp8": torch.float8_e4m3fn}[dtype]
+    view = {"f32": torch.float32, "bf16": torch.int16, "fp8": torch.uint8}[dtype]
+    scales = (0.5, 2.0) if dtype == "fp8" else None
+    lp = [i for i, t in enumerate(cg.tile_configs(elem))
+          if t[5] > 0 and t[6] == 0 and t[7] == 0 and cg.tile_supports(elem, i, cg.FWD, NHWC)]
+    assert lp
+    g = torch.Generator().manual_seed(91)
+    for si, s in enumerate(INPLACE_SHAPES):
+        d = cg.make_desc(cg.FWD, NHWC, s.b, s.h, s.w, s.c, s.kn, s.kh, s.kw, s.vpad, s.hpad, s.vstr, s.hstr)
+        W = (torch.randn(s.w_shape(NHWC), generator=g) * 0.5).to(tdt).view(view).to(DEV)
+        X = torch.randn(s.x_shape(NHWC), generator=g).to(tdt).view(view).to(DEV)
+        packed = torch.empty(cg.packed_weights_size(d, elem), dtype=torch.uint8, device=DEV)
+        cg.pack_weights(d, W, packed, dtype=elem)
+        for cfg in lp:
+            for splits in (1, 2):
+                print(f"in-place {dtype} shape {si} cfg {cfg} splits {splits}", flush=True)
+                y0 = torch.full(s.y_shape(NHWC), float("nan"), device=DEV)
+                y1 = torch.full(s.y_shape(NHWC), float("nan"), device=DEV)
+                cg.force_config(cfg, splits)
+                try:
+                    cg.conv_ex(d, W, X, y0, dtype=elem, scales=scales)
+                    torch.cuda.synchronize()
+                    plan = cg.last_plan()
+                    cg.conv_packed(d, packed, X, y1, dtype=elem, scales=scales)
+                    torch.cuda.synchronize()
+                    kplan = cg.last_plan()
+                finally:
+                    cg.force_config(-1, 0)
+                assert plan[0] == cfg and kplan[:3] == plan[:3], (plan, kplan)
+                assert torch.equal(y0, y1), (si, cfg, splits)
