@@ -66,6 +66,15 @@ def _run(cmd, verbose):
 # forward pass (NHWC for every type, NCHW for FP32).
 SLICE_TYPES = [("f32", "float"), ("bf16", "uint16_t"), ("fp8", "uint8_t")]
 
+# Per operand type extra flags.  FP8: without machine sinking.  The loads the
+# pipelined kernels issue with an out-of-range offset for a dead step become
+# uniform branches, and the pass then moved the FP8 MFMAs (32-byte operands)
+# across them, past the scheduling fences of their read groups, up to the
+# end of the loop body: every group's operands were live at once and the
+# larger tiles spilled (up to 300 VGPRs to scratch).  Without it no FP8
+# pipelined kernel spills; FP32 and BF16 keep their MFMAs in place.
+TYPE_FLAGS = {"fp8": ["-mllvm", "-disable-machine-sink"]}
+
 
 def slices():
     out = []
@@ -96,13 +105,15 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     slice_src = os.path.join(CSRC, "kernels", "slice.hip")
     for tn, ct, op, fam in slices():  # the largest translation units first
         o = os.path.join(BUILD, f"slice_{tn}_{op}_{fam}.o")
-        cmds.append([hipcc, *HIPCC_FLAGS, f"-DCG_SLICE_T={ct}", f"-DCG_SLICE_TN={tn}", f"-DCG_SLICE_OP={op}",
+        cmds.append([hipcc, *HIPCC_FLAGS, *TYPE_FLAGS.get(tn, []), f"-DCG_SLICE_T={ct}", f"-DCG_SLICE_TN={tn}",
+                     f"-DCG_SLICE_OP={op}",
                      f"-DCG_SLICE_FAM={fam}", "-c", slice_src, "-o", o])
         objs.append(o)
     single_src = os.path.join(CSRC, "kernels", "single.hip")
     for tn, ct in SLICE_TYPES:  # the single-launch batch kernel, one per operand type
         o = os.path.join(BUILD, f"single_{tn}.o")
-        cmds.append([hipcc, *HIPCC_FLAGS, f"-DCG_SINGLE_T={ct}", f"-DCG_SINGLE_TN={tn}", "-c", single_src, "-o", o])
+        cmds.append([hipcc, *HIPCC_FLAGS, *TYPE_FLAGS.get(tn, []), f"-DCG_SINGLE_T={ct}", f"-DCG_SINGLE_TN={tn}", "-c",
+                     single_src, "-o", o])
         objs.append(o)
     for s in hip_srcs:
         o = os.path.join(BUILD, os.path.basename(s) + ".o")
