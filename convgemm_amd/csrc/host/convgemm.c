@@ -568,11 +568,10 @@ static int weight_repack(const convgemm_desc_t *d, int elem, int *d0, int *d1, i
  * repack launch (about 4.5 us) but the kernel then reads its weight operand
  * with transposing LDS reads: measured on the same plans it ran 2-18%
  * (mean 8%) longer in BF16 and 0-5% (mean 3.5%) in FP32, so it pays below
- * roughly 24 (BF16) or 10 (FP32) GFLOP per call.  In FP8 the best plan of a
- * layer ran between 15% shorter and 7% longer than the best repacked one
- * (the repacked FP8 operand reads make the compiler spill on the larger
- * tiles), and reading in place paid or tied up to the largest layer
- * measured, 118 GFLOP. */
+ * roughly 24 (BF16) or 10 (FP32) GFLOP per call.  In FP8 the best in-place
+ * plan of a layer ran 1-22% longer than the best repacked one; with the
+ * repack launch counted, reading in place paid or tied up to the largest
+ * layer measured, 118 GFLOP. */
 static int raw_weights_ok(int elem, const cg_kparams *kp, const cg_tilecfg *t, const void *in) {
     if (kp->op != CG_FWD_NHWC || kp->ffd || !kp->vec || t->lp <= 0 || t->halo ||
         (kp->KN * esize_of(elem)) % 16 || (uintptr_t)in % 16)
