@@ -561,7 +561,7 @@ static int weight_repack(const convgemm_desc_t *d, int elem, int *d0, int *d1, i
 }
 
 /* Should the plan read the caller's weights in place (cg_kparams.braw)?  The
- * pipelined forward NHWC kernel can read [c][kh][kw][kn] rows of whole
+ * pipelined and halo-tile forward NHWC kernels can read [c][kh][kw][kn] rows of whole
  * 16-byte kn chunks from a 16-byte aligned tensor; weights already
  * prepacked, and the backward-data forward form (whose weights need their
  * taps reversed), keep the repacked order.  Reading in place saves the
@@ -571,12 +571,16 @@ static int weight_repack(const convgemm_desc_t *d, int elem, int *d0, int *d1, i
  * roughly 24 (BF16) or 10 (FP32) GFLOP per call.  In FP8 the best in-place
  * plan of a layer ran 1-22% longer than the best repacked one; with the
  * repack launch counted, reading in place paid or tied up to the largest
- * layer measured, 118 GFLOP. */
+ * layer measured, 118 GFLOP.  The halo-tile kernel streams only its weights
+ * through the pipeline: in place it ran 0-4% longer in FP32 and 0.1-4.6 us
+ * longer per call in BF16 and FP8 at 15 GFLOP (about the repack launch at
+ * worst), so its limit is 20 GFLOP in BF16 and FP8. */
 static int raw_weights_ok(int elem, const cg_kparams *kp, const cg_tilecfg *t, const void *in) {
-    if (kp->op != CG_FWD_NHWC || kp->ffd || !kp->vec || t->lp <= 0 || t->halo ||
+    if (kp->op != CG_FWD_NHWC || kp->ffd || !kp->vec || t->lp <= 0 ||
         (kp->KN * esize_of(elem)) % 16 || (uintptr_t)in % 16)
         return 0;
     const double flops = 2.0 * (double)kp->P * kp->C * kp->taps * kp->KN;
+    if (t->halo) return flops <= (elem == CG_ELEM_F32 ? 10e9 : 20e9);
     return flops <= (elem == CG_ELEM_FP8 ? 100e9 : elem == CG_ELEM_BF16 ? 24e9 : 10e9);
 }
 

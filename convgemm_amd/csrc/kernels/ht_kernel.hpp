@@ -28,7 +28,10 @@
 
 namespace cg {
 
-template <typename T, int BM, int BN, int WM, int WN, int NS, int HS>
+// RAWB: p.Bm holds the caller's [c][kh][kw][kn] weights, read in place as in
+// the pipelined kernel (raw_rows / raw_read): a weight stage is one row per
+// channel of the chunk, holding the tile's BN kn.
+template <typename T, int BM, int BN, int WM, int WN, int NS, int HS, bool RAWB = false>
 __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &p, const int lin) {
     static_assert(sizeof(T) == 4 || sizeof(T) == 2 || sizeof(T) == 1, "FP32, BF16 or FP8 operands");
     constexpr int ES = sizeof(T);
@@ -42,6 +45,10 @@ __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &
     static_assert((NW == 4 || NW == 8) && TR >= 1 && TC >= 1 && WTM % 32 == 0 && WTN % 32 == 0, "wave tile");
     static_assert(BN % (8 * NW) == 0 && HS % (8 * NW) == 0 && NS >= 2 && NS <= 4, "tile");
     static_assert(NI + (NS - 1) * IB < 64, "vmcnt");
+    constexpr int CPW = BN * ES / 16;  // RAWB: 16-byte chunks per weight row
+    static_assert(!RAWB || (CPW >= (ES == 4 ? 16 : ES == 2 ? 8 : 4) && 64 % CPW == 0 &&
+                            (RB / ES) / ((64 / CPW) * NW) == IB && (RB / ES) % ((64 / CPW) * NW) == 0),
+                  "raw weight rows");
 
     __shared__ __attribute__((aligned(1024))) char sh[HS * RB];
     __shared__ __attribute__((aligned(1024))) char sw0[WST];
@@ -100,17 +107,21 @@ __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &
         const bool ok = s < slots && (unsigned)ih < (unsigned)p.H && (unsigned)iw < (unsigned)p.W;
         h_off[k] = ok ? (unsigned)(((b * p.H + ih) * p.W + iw) * CB + h_cb[k]) : OOB;
     }
-    // ---- weight rows ([kn][tap][c] repack): as in the pipelined kernel ----
+    // ---- weight rows ([kn][tap][c] repack, or in place): as in the pipelined kernel ----
     int b_off[IB], b_cb[IB];
     bool b_ok[IB];
+    if constexpr (RAWB) {
+        raw_rows<ES, BN, IB>(p, wave, lane, nt, b_off, b_cb, b_ok);
+    } else {
 #pragma unroll
-    for (int j = 0; j < IB; j++) {
-        const int r = (wave * IB + j) * 8 + lrow;
-        const int chunk = lpos ^ ((r >> 1) & 7);
-        b_cb[j] = chunk * 16;
-        const int n = nt * BN + r;
-        b_ok[j] = n < p.KN;
-        b_off[j] = n * p.taps * CB + b_cb[j];
+        for (int j = 0; j < IB; j++) {
+            const int r = (wave * IB + j) * 8 + lrow;
+            const int chunk = lpos ^ ((r >> 1) & 7);
+            b_cb[j] = chunk * 16;
+            const int n = nt * BN + r;
+            b_ok[j] = n < p.KN;
+            b_off[j] = n * p.taps * CB + b_cb[j];
+        }
     }
 
     // ---- reduction range: split-K divides the channel chunks ----
@@ -139,8 +150,15 @@ __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &
         const int tapB = i_tap * CB + i_cb * RB;
 #pragma unroll
         for (int j = J0; j < J1; j++) {
-            const bool ok = live && b_ok[j] && b_cb[j] < lim;
-            const unsigned off = ok ? (unsigned)(b_off[j] + tapB) : OOB;
+            bool ok;
+            unsigned off;
+            if constexpr (RAWB) {
+                ok = live && b_ok[j] && raw_live<ES>(p, b_cb[j], i_cb);
+                off = ok ? raw_off<ES>(p, b_off[j], i_cb, i_tap) : OOB;
+            } else {
+                ok = live && b_ok[j] && b_cb[j] < lim;
+                off = ok ? (unsigned)(b_off[j] + tapB) : OOB;
+            }
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rB, (lds_void_t *)(uintptr_t)(base + (wave * IB + j) * 1024), 16,
                                                      off, 0, 0, 0);
         }
@@ -172,6 +190,8 @@ __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &
     const int l32 = lane & 31, h = lane >> 5;
     const int swz_b = (l32 >> 1) & 7;  // weight rows of this lane are l32 mod 32
     const int rowB = (wn * WTN + l32) * RB;
+    int woff[TC];  // RAWB: this lane's offsets in a weight stage
+    if constexpr (RAWB) raw_woff<ES, BN, TC>(lane, wn * WTN, woff);
     int aslot[TR];  // halo slot of this lane's output pixel in each 32-row block (tap 0)
 #pragma unroll
     for (int a = 0; a < TR; a++) {
@@ -204,8 +224,12 @@ __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &
                 r.a[a] = *(const i32x4 *)(sh + sl * RB + (((2 * g + h) ^ ((sl >> 1) & 7)) * 16));
             }
             const int co = ((2 * g + h) ^ swz_b) * 16;
+            if constexpr (RAWB) {
+                raw_read<ES, BN, TC>(wb, woff, g, r.b);
+            } else {
 #pragma unroll
-            for (int c = 0; c < TC; c++) r.b[c] = *(const i32x4 *)(wb + rowB + c * 32 * RB + co);
+                for (int c = 0; c < TC; c++) r.b[c] = *(const i32x4 *)(wb + rowB + c * 32 * RB + co);
+            }
         } else {
             const int c0 = 4 * g + 2 * h;
 #pragma unroll
@@ -217,11 +241,15 @@ __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &
                 r.a[a] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
             }
             const int co0 = (c0 ^ swz_b) * 16, co1 = ((c0 + 1) ^ swz_b) * 16;
+            if constexpr (RAWB) {
+                raw_read<ES, BN, TC>(wb, woff, g, r.b);
+            } else {
 #pragma unroll
-            for (int c = 0; c < TC; c++) {
-                const i32x4 lo = *(const i32x4 *)(wb + rowB + c * 32 * RB + co0);
-                const i32x4 hi = *(const i32x4 *)(wb + rowB + c * 32 * RB + co1);
-                r.b[c] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+                for (int c = 0; c < TC; c++) {
+                    const i32x4 lo = *(const i32x4 *)(wb + rowB + c * 32 * RB + co0);
+                    const i32x4 hi = *(const i32x4 *)(wb + rowB + c * 32 * RB + co1);
+                    r.b[c] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+                }
             }
         }
     };
@@ -416,9 +444,9 @@ __device__ __attribute__((always_inline)) inline void ht_tile(const cg_kparams &
     if (s_epi) __syncthreads();  // the LDS is rewritten by the next item of a persistent launch
 }
 
-template <typename T, int BM, int BN, int WM, int WN, int NS, int HS>
+template <typename T, int BM, int BN, int WM, int WN, int NS, int HS, bool RAWB>
 __global__ void __launch_bounds__(WM * WN * 64) ht_kernel(const cg_kparams p) {
-    ht_tile<T, BM, BN, WM, WN, NS, HS>(p, xcd_run(blockIdx.x, gridDim.x));
+    ht_tile<T, BM, BN, WM, WN, NS, HS, RAWB>(p, xcd_run(blockIdx.x, gridDim.x));
 }
 
 // Grouped form, with the item layout of igemm_grouped_kernel.
@@ -441,7 +469,8 @@ __global__ void __launch_bounds__(WM * WN * 64) ht_grouped_kernel(const cg_kpara
     }
 }
 
-// Forward NHWC with 16-byte channel chunks; no stream-K form.
+// Forward NHWC with 16-byte channel chunks; no stream-K form.  kp->braw: the
+// weights are the caller's [c][kh][kw][kn] (plain launches).
 template <typename T, int BM, int BN, int WM, int WN, int NS, int HS>
 inline int launch_ht(const cg_kparams *kp, const cg_group *g, int *occ, hipStream_t st) {
     constexpr int NT = WM * WN * 64;
@@ -454,7 +483,7 @@ inline int launch_ht(const cg_kparams *kp, const cg_group *g, int *occ, hipStrea
         return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(
             occ, reinterpret_cast<const void *>(&ht_grouped_kernel<T, BM, BN, WM, WN, NS, HS>), NT, 0);
     }
-    if (kp->sk > 0) return (int)hipErrorInvalidValue;
+    if (kp->sk > 0 || (kp->braw && g)) return (int)hipErrorInvalidValue;
     if (g) {
         if (g->total <= 0) return 0;
         hipLaunchKernelGGL((ht_grouped_kernel<T, BM, BN, WM, WN, NS, HS>), dim3((unsigned)g->grid), dim3(NT), 0, st,
@@ -462,7 +491,11 @@ inline int launch_ht(const cg_kparams *kp, const cg_group *g, int *occ, hipStrea
         return (int)hipGetLastError();
     }
     const long long nb = (long long)kp->mtiles * kp->ntiles * kp->splits;
-    hipLaunchKernelGGL((ht_kernel<T, BM, BN, WM, WN, NS, HS>), dim3((unsigned)nb), dim3(NT), 0, st, *kp);
+    if (kp->braw) {
+        hipLaunchKernelGGL((ht_kernel<T, BM, BN, WM, WN, NS, HS, true>), dim3((unsigned)nb), dim3(NT), 0, st, *kp);
+        return (int)hipGetLastError();
+    }
+    hipLaunchKernelGGL((ht_kernel<T, BM, BN, WM, WN, NS, HS, false>), dim3((unsigned)nb), dim3(NT), 0, st, *kp);
     return (int)hipGetLastError();
 }
 

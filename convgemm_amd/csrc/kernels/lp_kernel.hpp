@@ -85,6 +85,106 @@ __device__ __forceinline__ int rw_swz(int r) {
     else return ((r >> 1) & 1) << 2;
 }
 
+// In-place weights, load side: instruction j of a wave fills the weight rows
+// (wave * IB + j) * PIW + lane / CPW of a stage (PIW rows of CPW 16-byte
+// chunks per instruction).  b_cb[j] is the row (the channel within the K
+// step), b_off[j] the byte offset of its chunk without the K step's part.
+template <int ES, int BN, int IB>
+__device__ __forceinline__ void raw_rows(const cg_kparams &p, int wave, int lane, int nt, int (&b_off)[IB],
+                                         int (&b_cb)[IB], bool (&b_ok)[IB]) {
+    constexpr int CPW = BN * ES / 16, PIW = 64 / CPW;
+#pragma unroll
+    for (int j = 0; j < IB; j++) {
+        const int r = (wave * IB + j) * PIW + lane / CPW;
+        const int chunk = (lane % CPW) ^ rw_swz<ES, CPW>(r);
+        const int n = nt * BN + chunk * (16 / ES);
+        b_cb[j] = r;
+        b_ok[j] = n < p.KN;
+        b_off[j] = (int)(((unsigned)r * (unsigned)p.taps * (unsigned)p.KN + (unsigned)n) * ES);
+    }
+}
+
+// Is row b_cb of K step (channel block cb) a channel (rows past C read 0)?
+template <int ES>
+__device__ __forceinline__ bool raw_live(const cg_kparams &p, int b_cb, int cb) {
+    return b_cb < p.C - cb * (128 / ES);
+}
+
+// Global byte offset of the in-place weight load with base b_off in K step (cb, tap).
+template <int ES>
+__device__ __forceinline__ unsigned raw_off(const cg_kparams &p, int b_off, int cb, int tap) {
+    return (unsigned)b_off + ((unsigned)(cb * (128 / ES)) * (unsigned)p.taps + (unsigned)tap) * (unsigned)p.KN * ES;
+}
+
+// In-place weights, read side: this lane's byte offset, within a weight
+// stage, for each 32-column block b of the wave's columns col0 + 32b ..
+template <int ES, int BN, int TC>
+__device__ __forceinline__ void raw_woff(int lane, int col0, int (&woff)[TC]) {
+    constexpr int RWB = BN * ES, CPW = RWB / 16;
+    if constexpr (ES == 2) {
+        const int G = lane >> 4, s16 = lane & 15;
+        const int kl = 8 * (G >> 1) + (s16 >> 2);  // first row of the group's first read
+        const int ml = 16 * (G & 1) + 4 * (s16 & 3);
+        const int sw = rw_swz<ES, CPW>(kl);       // == that of kl + 4 and kl + 16g
+#pragma unroll
+        for (int b = 0; b < TC; b++) {
+            const int n = col0 + b * 32 + ml;
+            woff[b] = kl * RWB + (((n >> 3) ^ sw) << 4) + (n & 7) * 2;
+        }
+    } else if constexpr (ES == 1) {
+        const int G = lane >> 4, s16 = lane & 15;
+        const int kl = 32 * (G >> 1) + (s16 >> 1);  // row of read 0 of group 0
+        const int sw = rw_swz<ES, CPW>(kl);         // == that of kl + 64g + 8q
+#pragma unroll
+        for (int b = 0; b < TC; b++) {
+            const int c = ((col0 + b * 32) >> 4) + (G & 1);
+            woff[b] = kl * RWB + ((c ^ sw) << 4) + 8 * (s16 & 1);
+        }
+    } else {
+        const int l32 = lane & 31, h = lane >> 5;
+#pragma unroll
+        for (int b = 0; b < TC; b++) {
+            const int n = col0 + b * 32 + l32;
+            woff[b] = 4 * h * RWB + (((n >> 2) ^ (h << 3)) << 4) + (n & 3) * 4;
+        }
+    }
+}
+
+// The MFMA B operands of read group g from a weight stage at wst (the K
+// order of the activations' operand).
+template <int ES, int BN, int TC, typename OpV>
+__device__ __forceinline__ void raw_read(const char *wst, const int (&woff)[TC], int g, OpV (&b)[TC]) {
+    constexpr int RWB = BN * ES;
+    if constexpr (ES == 2) {
+        auto tr = [&](int off) __attribute__((always_inline)) -> i32x2 {
+            typedef short s16x4 __attribute__((ext_vector_type(4)));
+            typedef __attribute__((address_space(3))) s16x4 lds_s16x4;
+            return __builtin_bit_cast(i32x2, __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4 *)(wst + off)));
+        };
+#pragma unroll
+        for (int c = 0; c < TC; c++) {
+            const i32x2 lo = tr(woff[c] + 16 * g * RWB), hi = tr(woff[c] + (16 * g + 4) * RWB);
+            b[c] = i32x4{lo[0], lo[1], hi[0], hi[1]};
+        }
+    } else if constexpr (ES == 1) {
+        auto tr = [&](int off) __attribute__((always_inline)) -> i32x2 {
+            typedef __attribute__((address_space(3))) i32x2 lds_i32x2;
+            return __builtin_amdgcn_ds_read_tr8_b64_v2i32((lds_i32x2 *)(wst + off));
+        };
+#pragma unroll
+        for (int c = 0; c < TC; c++) {
+            const int o = woff[c] + 64 * g * RWB;
+            const i32x2 q0 = tr(o), q1 = tr(o + 8 * RWB), q2 = tr(o + 16 * RWB), q3 = tr(o + 24 * RWB);
+            b[c] = i32x8{q0[0], q0[1], q1[0], q1[1], q2[0], q2[1], q3[0], q3[1]};
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < TC; c++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) b[c][j] = *(const int *)(wst + woff[c] + (8 * g + j) * RWB);
+    }
+}
+
 template <typename T, int BM, int BN, int WM, int WN, int NS, bool EXT = false, bool RAWB = false>
 __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &p, const int lin, const int sk_k0 = -1,
                                                          const int sk_k1 = 0, float *piece = nullptr,
@@ -151,16 +251,8 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
     constexpr int PIW = 64 / CPW;  // RAWB: weight rows per load instruction
     static_assert(!RAWB || (CPW >= (ES == 4 ? 16 : ES == 2 ? 8 : 4) && 64 % CPW == 0 && KSW / (PIW * NW) == IB &&
                             KSW % (PIW * NW) == 0), "raw weight rows");
-    if constexpr (RAWB) {  // row r = channel (step base + r), b_off = its kn chunk's byte offset without the step's
-#pragma unroll
-        for (int j = 0; j < IB; j++) {
-            const int r = (wave * IB + j) * PIW + lane / CPW;
-            const int chunk = (lane % CPW) ^ rw_swz<ES, CPW>(r);
-            const int n = nt * BN + chunk * (16 / ES);
-            b_cb[j] = r;
-            b_ok[j] = n < p.KN;
-            b_off[j] = (int)(((unsigned)r * (unsigned)p.taps * (unsigned)p.KN + (unsigned)n) * ES);
-        }
+    if constexpr (RAWB) {
+        raw_rows<ES, BN, IB>(p, wave, lane, nt, b_off, b_cb, b_ok);
     } else {
 #pragma unroll
         for (int j = 0; j < IB; j++) {
@@ -228,11 +320,9 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
                 const int j = jj - IA;
                 bool ok;
                 unsigned off;
-                if constexpr (RAWB) {  // rows past C read 0
-                    ok = live && b_ok[j] && b_cb[j] < p.C - i_cb * KSW;
-                    off = ok ? (unsigned)b_off[j] +
-                                   ((unsigned)(i_cb * KSW) * (unsigned)p.taps + (unsigned)i_tap) * (unsigned)p.KN * ES
-                             : OOB;
+                if constexpr (RAWB) {
+                    ok = live && b_ok[j] && raw_live<ES>(p, b_cb[j], i_cb);
+                    off = ok ? raw_off<ES>(p, b_off[j], i_cb, i_tap) : OOB;
                 } else {
                     ok = live && b_ok[j] && b_cb[j] < lim;
                     off = ok ? (unsigned)(b_off[j] + tapB) : OOB;
@@ -276,54 +366,15 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
     };
     // RAWB: per-lane byte offset of this lane's weight elements in each 32-column block
     int woff[TC];
-    if constexpr (RAWB && ES == 2) {
-        const int G = lane >> 4, s16 = lane & 15;
-        const int kl = 8 * (G >> 1) + (s16 >> 2);  // first row of the group's first read
-        const int ml = 16 * (G & 1) + 4 * (s16 & 3);
-        const int sw = rw_swz<ES, CPW>(kl);       // == that of kl + 4 and kl + 16g
-#pragma unroll
-        for (int b = 0; b < TC; b++) {
-            const int n = wn * WTN + b * 32 + ml;
-            woff[b] = BM * RB + kl * RWB + (((n >> 3) ^ sw) << 4) + (n & 7) * 2;
-        }
-    } else if constexpr (RAWB && ES == 1) {
-        const int G = lane >> 4, s16 = lane & 15;
-        const int kl = 32 * (G >> 1) + (s16 >> 1);  // row of read 0 of group 0
-        const int sw = rw_swz<ES, CPW>(kl);         // == that of kl + 64g + 8q
-#pragma unroll
-        for (int b = 0; b < TC; b++) {
-            const int c = ((wn * WTN + b * 32) >> 4) + (G & 1);
-            woff[b] = BM * RB + kl * RWB + ((c ^ sw) << 4) + 8 * (s16 & 1);
-        }
-    } else if constexpr (RAWB) {
-#pragma unroll
-        for (int b = 0; b < TC; b++) {
-            const int n = wn * WTN + b * 32 + l32;
-            woff[b] = BM * RB + 4 * h * RWB + (((n >> 2) ^ (h << 3)) << 4) + (n & 3) * 4;
-        }
-    }
+    if constexpr (RAWB) raw_woff<ES, BN, TC>(lane, wn * WTN, woff);
     auto ldg = [&](auto s, int g, Ops &r) __attribute__((always_inline)) {
         const char *base = stage_ptr(s);
         if constexpr (ES != 1) {
             const int co = ((2 * g + h) ^ swz) * 16;
 #pragma unroll
             for (int a = 0; a < TR; a++) r.a[a] = *(const i32x4 *)(base + rowA + a * 32 * RB + co);
-            if constexpr (RAWB && ES == 2) {
-                auto tr = [&](int off) __attribute__((always_inline)) -> i32x2 {
-                    typedef short s16x4 __attribute__((ext_vector_type(4)));
-                    typedef __attribute__((address_space(3))) s16x4 lds_s16x4;
-                    return __builtin_bit_cast(i32x2, __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4 *)(base + off)));
-                };
-#pragma unroll
-                for (int b = 0; b < TC; b++) {
-                    const i32x2 lo = tr(woff[b] + 16 * g * RWB), hi = tr(woff[b] + (16 * g + 4) * RWB);
-                    r.b[b] = i32x4{lo[0], lo[1], hi[0], hi[1]};
-                }
-            } else if constexpr (RAWB) {
-#pragma unroll
-                for (int b = 0; b < TC; b++)
-#pragma unroll
-                    for (int j = 0; j < 4; j++) r.b[b][j] = *(const int *)(base + woff[b] + (8 * g + j) * RWB);
+            if constexpr (RAWB) {
+                raw_read<ES, BN, TC>(base + BM * RB, woff, g, r.b);
             } else {
 #pragma unroll
                 for (int b = 0; b < TC; b++) r.b[b] = *(const i32x4 *)(base + rowB + b * 32 * RB + co);
@@ -338,16 +389,7 @@ __device__ __attribute__((always_inline)) inline void lp_tile(const cg_kparams &
                 r.a[a] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
             }
             if constexpr (RAWB) {
-                auto tr = [&](int off) __attribute__((always_inline)) -> i32x2 {
-                    typedef __attribute__((address_space(3))) i32x2 lds_i32x2;
-                    return __builtin_amdgcn_ds_read_tr8_b64_v2i32((lds_i32x2 *)(base + off));
-                };
-#pragma unroll
-                for (int b = 0; b < TC; b++) {
-                    const int o = woff[b] + 64 * g * RWB;
-                    const i32x2 q0 = tr(o), q1 = tr(o + 8 * RWB), q2 = tr(o + 16 * RWB), q3 = tr(o + 24 * RWB);
-                    r.b[b] = i32x8{q0[0], q0[1], q1[0], q1[1], q2[0], q2[1], q3[0], q3[1]};
-                }
+                raw_read<ES, BN, TC>(base + BM * RB, woff, g, r.b);
             } else {
 #pragma unroll
                 for (int b = 0; b < TC; b++) {
